@@ -1,0 +1,243 @@
+/*
+ * rtg.h — C ABI of the MI355X path-tracing back end (librtg.so).
+ *
+ * The library renders and traces a compiled scene on one gfx950 GPU. The
+ * host side (JSON scene load, Scene::compile, binned-SAH BVH build) stays in
+ * C++ and hands the compiled scene over as plain arrays, field for field as
+ * the reference's `Scene` holds them (include/raytracer/core/scene.hpp:33-56).
+ * No C++ exception crosses this boundary: every entry point returns an
+ * rtg_status and rtg_last_error() gives the message.
+ *
+ * Threading: one context per host thread. Calls on one context are
+ * serialised on that context's HIP stream; contexts on the same device run
+ * concurrently on their own streams.
+ */
+#ifndef RTG_H
+#define RTG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTG_ABI_VERSION 1
+
+typedef int32_t rtg_status;
+enum {
+  RTG_OK = 0,
+  RTG_ERR_INVALID_ARG = -1,   /* bad argument or inconsistent scene arrays   */
+  RTG_ERR_OUT_OF_MEMORY = -2, /* device allocation failed / admission refused */
+  RTG_ERR_HIP = -3,           /* a HIP runtime call failed                    */
+  RTG_ERR_CANCELLED = -4,     /* the cancel flag was raised during the call  */
+  RTG_ERR_UNSUPPORTED = -5,   /* feature not available on this device/build  */
+  RTG_ERR_NO_DEVICE = -6      /* no HIP device                               */
+};
+
+/* ------------------------------------------------------------------------ */
+/* Compiled scene, as the reference's Scene::compile produces it (all FP64). */
+
+typedef struct rtg_vec3 { double x, y, z; } rtg_vec3;
+
+/* geometry/bvh.hpp:20-26 (BvhNode): DFS order, left child = index + 1,
+ * interior `offset` = right child, leaf `offset` = first prim_indices slot. */
+typedef struct rtg_bvh_node {
+  rtg_vec3 lo, hi;
+  uint32_t offset;
+  uint16_t count; /* 0 = interior */
+  uint8_t axis;
+  uint8_t pad;
+} rtg_bvh_node;
+
+enum { RTG_PRIM_SPHERE = 0, RTG_PRIM_TRIANGLE = 1 };
+/* core/scene.hpp:17 (Primitive = variant<Sphere, Triangle>),
+ * geometry/shapes.hpp:11-14 (Sphere), :101-105 (Triangle).
+ * Sphere: v0 = centre, v1.x = radius. Triangle: v0, v1, v2 and, when
+ * has_vertex_normals, n0, n1, n2. */
+typedef struct rtg_primitive {
+  uint32_t kind;
+  uint32_t material;
+  uint32_t has_vertex_normals;
+  uint32_t pad;
+  rtg_vec3 v0, v1, v2;
+  rtg_vec3 n0, n1, n2;
+} rtg_primitive;
+
+/* geometry/shapes.hpp:77-80 (Plane): unit normal, tested outside the BVH. */
+typedef struct rtg_plane {
+  rtg_vec3 point, normal;
+  uint32_t material, pad;
+} rtg_plane;
+
+enum { RTG_MAT_LAMBERTIAN = 0, RTG_MAT_METAL = 1, RTG_MAT_DIELECTRIC = 2, RTG_MAT_EMISSIVE = 3 };
+/* materials/material.hpp:14-56. c0: Lambertian albedo a | Metal albedo |
+ * Dielectric tint | Emissive colour; c1: Lambertian albedo2 (checker b);
+ * p0: checker_scale | roughness | ior | intensity. */
+typedef struct rtg_material {
+  uint32_t kind;
+  uint32_t two_sided;
+  rtg_vec3 c0, c1;
+  double p0;
+} rtg_material;
+
+/* core/scene.hpp:19-29 (PointLight, LightRef). */
+typedef struct rtg_point_light { rtg_vec3 position, intensity; } rtg_point_light;
+enum { RTG_LIGHT_POINT = 0, RTG_LIGHT_EMITTER = 1 };
+typedef struct rtg_light_ref { uint32_t kind, index; } rtg_light_ref;
+
+/* core/camera.hpp:29-51 after setup(): what generate() reads. */
+typedef struct rtg_camera {
+  rtg_vec3 origin, corner, du, dv, u, v, w;
+  double lens_radius;
+} rtg_camera;
+
+enum { RTG_BG_GRADIENT = 0, RTG_BG_SOLID = 1 };
+/* core/scene_desc.hpp:35-42 (Background). */
+typedef struct rtg_background {
+  uint32_t kind, pad;
+  rtg_vec3 top, bottom;
+  double intensity;
+} rtg_background;
+
+typedef struct rtg_scene_desc {
+  const rtg_primitive* prims;
+  uint32_t n_prims;
+  const uint8_t* is_sampled_light; /* n_prims entries */
+  const rtg_plane* planes;
+  uint32_t n_planes;
+  const rtg_material* materials;
+  uint32_t n_materials;
+  const rtg_point_light* point_lights;
+  uint32_t n_point_lights;
+  const rtg_light_ref* lights;
+  uint32_t n_lights;
+  const rtg_bvh_node* nodes;
+  uint32_t n_nodes;
+  const uint32_t* prim_indices;
+  uint32_t n_prim_indices;
+  rtg_camera camera;
+  rtg_background background;
+} rtg_scene_desc;
+
+/* ------------------------------------------------------------------------ */
+/* Render settings: the per-sample part of RenderSettings
+ * (core/scene_desc.hpp:20-35) plus the device precision mode. */
+
+enum { RTG_INTEGRATOR_PATH = 0, RTG_INTEGRATOR_DIRECT = 1, RTG_INTEGRATOR_NORMALS = 2 };
+/* RTG_PRECISION_F32: float arithmetic, stateless counter-hash RNG keyed on
+ *   (seed, pixel, sample, bounce, dimension) — the production path.
+ * RTG_PRECISION_F64: double arithmetic and the reference's own per-sample
+ *   PCG32 stream (math/rng.hpp:29-32) — reproduces the CPU renderer's
+ *   samples up to the last-bit differences of device sin/cos. */
+enum { RTG_PRECISION_F32 = 0, RTG_PRECISION_F64 = 1 };
+
+typedef struct rtg_render_settings {
+  int32_t width, height;
+  int32_t max_depth;
+  int32_t rr_min_depth;
+  int32_t integrator;
+  int32_t light_sampling;
+  int32_t use_bvh;
+  int32_t precision;
+  uint64_t seed;
+} rtg_render_settings;
+
+/* PathStats (rendering/integrator.hpp:11-16), TraversalCounters
+ * (geometry/bvh.hpp:34-37) with the same counting rules. */
+typedef struct rtg_stats {
+  uint64_t camera_samples;
+  uint64_t rays;          /* closest-hit rays (camera + bounces) */
+  uint64_t shadow_rays;   /* any-hit visibility rays actually cast */
+  uint64_t rr_terminated;
+  uint64_t invalid_samples;
+  uint64_t node_visits;   /* BVH node box tests */
+  uint64_t prim_tests;    /* primitive intersection tests */
+  double elapsed_ms;      /* device time of the call (HIP events) */
+} rtg_stats;
+
+/* ------------------------------------------------------------------------ */
+typedef struct rtg_context rtg_context;
+typedef struct rtg_scene rtg_scene;
+typedef struct rtg_accum rtg_accum;
+typedef struct rtg_buffer rtg_buffer;
+
+/* Opens `device` (HIP ordinal) and creates a stream. */
+rtg_status rtg_open(int device, rtg_context** out);
+rtg_status rtg_close(rtg_context* ctx);
+/* Message of the last failed call on `ctx` (ctx-owned; NULL ctx = global). */
+const char* rtg_last_error(const rtg_context* ctx);
+int rtg_abi_version(void);
+/* Device name and size (multiprocessors, HBM bytes). */
+rtg_status rtg_device_info(rtg_context* ctx, char* name, size_t name_len, int* multiprocessors,
+                           uint64_t* hbm_bytes);
+
+/* Uploads a compiled scene: flattens the BVH into 64-byte two-child records
+ * (float bounds rounded outward) and the primitives into leaf order.
+ * Replaces the scene argument of Renderer (rendering/renderer.hpp:52). */
+rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* desc, rtg_scene** out);
+rtg_status rtg_scene_release(rtg_scene* scene);
+/* Device bytes held by the scene (admission by HBM footprint). */
+uint64_t rtg_scene_bytes(const rtg_scene* scene);
+
+/* Accumulation buffer: per-pixel FP64 sum + u32 count (the reference
+ * Framebuffer, rendering/framebuffer.hpp:16-50), resident on the device. */
+rtg_status rtg_accum_create(rtg_context* ctx, int32_t width, int32_t height, rtg_accum** out);
+rtg_status rtg_accum_release(rtg_accum* acc);
+rtg_status rtg_accum_clear(rtg_context* ctx, rtg_accum* acc);
+/* Copies the buffer to host memory: sum = width*height*3 doubles (RGB,
+ * row-major, top row first), count = width*height u32. Either may be NULL. */
+rtg_status rtg_accum_read(rtg_context* ctx, const rtg_accum* acc, double* sum, uint32_t* count);
+uint32_t rtg_accum_min_samples(rtg_context* ctx, const rtg_accum* acc);
+
+/* Renders samples [s0, s1) of every pixel into `acc` — one progressive pass
+ * of Renderer::render (rendering/renderer.cpp:58-105) with render_tile's
+ * per-sample loop (:21-56). `cancel`, if non-NULL, is polled between device
+ * chunks; a raised flag returns RTG_ERR_CANCELLED with whole chunks kept. */
+rtg_status rtg_render_pass(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* settings,
+                           uint32_t s0, uint32_t s1, rtg_accum* acc, const volatile int32_t* cancel,
+                           rtg_stats* stats);
+
+/* Tone map + sRGB encode on the device (rendering/tonemap.cpp:14-53):
+ * writes width*height*3 bytes. tone_map: 0 clamp, 1 reinhard, 2 aces. */
+rtg_status rtg_resolve(rtg_context* ctx, const rtg_accum* acc, int32_t tone_map, double exposure, uint8_t* rgb);
+
+/* ------------------------------------------------------------------------ */
+/* Batch ray queries: Scene::intersect / occluded / intersect_brute
+ * (src/core/scene.cpp:176-234) over n rays.
+ * Ray record: 8 doubles {ox, oy, oz, dx, dy, dz, tmin, tmax}.
+ * Hit record (rtg_hit): t, primitive index (or plane index) and flags. */
+typedef struct rtg_hit {
+  double t;
+  uint32_t prim;  /* 0xFFFFFFFF = miss */
+  uint32_t flags; /* bit0 hit, bit1 is_plane, bit2 front_face */
+} rtg_hit;
+
+enum { RTG_TRACE_CLOSEST = 0, RTG_TRACE_ANY = 1 };
+typedef struct rtg_trace_opts {
+  int32_t mode;      /* RTG_TRACE_CLOSEST | RTG_TRACE_ANY */
+  int32_t use_bvh;   /* 0 = brute force over all primitives */
+  int32_t precision; /* RTG_PRECISION_F32 | RTG_PRECISION_F64 */
+  int32_t pad;
+} rtg_trace_opts;
+
+/* Device buffers for resident inputs/outputs. */
+rtg_status rtg_buffer_create(rtg_context* ctx, uint64_t bytes, rtg_buffer** out);
+rtg_status rtg_buffer_release(rtg_buffer* buf);
+rtg_status rtg_buffer_write(rtg_context* ctx, rtg_buffer* buf, const void* src, uint64_t bytes);
+rtg_status rtg_buffer_read(rtg_context* ctx, const rtg_buffer* buf, void* dst, uint64_t bytes);
+
+/* Traces n rays held in `rays` (n*8 doubles) into `hits` (n rtg_hit). For
+ * RTG_TRACE_ANY, hits[i].flags bit0 = occluded in (tmin, tmax). */
+rtg_status rtg_trace(rtg_context* ctx, const rtg_scene* scene, const rtg_buffer* rays, uint64_t n,
+                     const rtg_trace_opts* opts, rtg_buffer* hits, rtg_stats* stats);
+/* Same, with host arrays (copies in and out; PCIe time included). */
+rtg_status rtg_trace_host(rtg_context* ctx, const rtg_scene* scene, const double* rays, uint64_t n,
+                          const rtg_trace_opts* opts, rtg_hit* hits, rtg_stats* stats);
+
+rtg_status rtg_synchronize(rtg_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTG_H */

@@ -1,0 +1,301 @@
+"""ctypes binding of the device back end's C ABI (include/rtg.h, librtg.so)
+and of the host library (include/rt_host.h, librthost.so).
+
+The structures mirror rtg.h field for field. Loading fails loudly when a
+library is missing: there is no CPU fallback for any entry point here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+
+
+class Vec3(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+
+class BvhNode(C.Structure):
+    _fields_ = [("lo", Vec3), ("hi", Vec3), ("offset", C.c_uint32), ("count", C.c_uint16),
+                ("axis", C.c_uint8), ("pad", C.c_uint8)]
+
+
+class Primitive(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("material", C.c_uint32), ("has_vertex_normals", C.c_uint32),
+                ("pad", C.c_uint32), ("v0", Vec3), ("v1", Vec3), ("v2", Vec3),
+                ("n0", Vec3), ("n1", Vec3), ("n2", Vec3)]
+
+
+class Plane(C.Structure):
+    _fields_ = [("point", Vec3), ("normal", Vec3), ("material", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class Material(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("two_sided", C.c_uint32), ("c0", Vec3), ("c1", Vec3), ("p0", C.c_double)]
+
+
+class PointLight(C.Structure):
+    _fields_ = [("position", Vec3), ("intensity", Vec3)]
+
+
+class LightRef(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("index", C.c_uint32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("origin", Vec3), ("corner", Vec3), ("du", Vec3), ("dv", Vec3), ("u", Vec3), ("v", Vec3),
+                ("w", Vec3), ("lens_radius", C.c_double)]
+
+
+class Background(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("pad", C.c_uint32), ("top", Vec3), ("bottom", Vec3),
+                ("intensity", C.c_double)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("prims", C.POINTER(Primitive)), ("n_prims", C.c_uint32),
+                ("is_sampled_light", C.POINTER(C.c_uint8)),
+                ("planes", C.POINTER(Plane)), ("n_planes", C.c_uint32),
+                ("materials", C.POINTER(Material)), ("n_materials", C.c_uint32),
+                ("point_lights", C.POINTER(PointLight)), ("n_point_lights", C.c_uint32),
+                ("lights", C.POINTER(LightRef)), ("n_lights", C.c_uint32),
+                ("nodes", C.POINTER(BvhNode)), ("n_nodes", C.c_uint32),
+                ("prim_indices", C.POINTER(C.c_uint32)), ("n_prim_indices", C.c_uint32),
+                ("camera", Camera), ("background", Background)]
+
+
+class RenderSettings(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("max_depth", C.c_int32),
+                ("rr_min_depth", C.c_int32), ("integrator", C.c_int32), ("light_sampling", C.c_int32),
+                ("use_bvh", C.c_int32), ("precision", C.c_int32), ("seed", C.c_uint64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("camera_samples", C.c_uint64), ("rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("rr_terminated", C.c_uint64), ("invalid_samples", C.c_uint64), ("node_visits", C.c_uint64),
+                ("prim_tests", C.c_uint64), ("elapsed_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Hit(C.Structure):
+    _fields_ = [("t", C.c_double), ("prim", C.c_uint32), ("flags", C.c_uint32)]
+
+
+HIT_DTYPE = np.dtype([("t", "<f8"), ("prim", "<u4"), ("flags", "<u4")])
+
+
+class TraceOpts(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("use_bvh", C.c_int32), ("precision", C.c_int32), ("pad", C.c_int32)]
+
+
+PRECISION_F32, PRECISION_F64 = 0, 1
+INTEGRATORS = {"path": 0, "direct": 1, "normals": 2}
+TONE_MAPS = {"clamp": 0, "reinhard": 1, "aces": 2}
+
+_VP = C.c_void_p
+
+
+def _lib_path(name):
+    return os.path.join(LIB_DIR, name)
+
+
+class RtgError(RuntimeError):
+    pass
+
+
+_rtg = None
+
+
+def load_rtg():
+    """Loads librtg.so (the HIP back end); raises if it was not built."""
+    global _rtg
+    if _rtg is not None:
+        return _rtg
+    path = _lib_path("librtg.so")
+    if not os.path.exists(path):
+        raise RtgError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    sig = {
+        "rtg_open": (C.c_int32, [C.c_int, P(_VP)]),
+        "rtg_close": (C.c_int32, [_VP]),
+        "rtg_last_error": (C.c_char_p, [_VP]),
+        "rtg_abi_version": (C.c_int, []),
+        "rtg_device_info": (C.c_int32, [_VP, C.c_char_p, C.c_size_t, P(C.c_int), P(C.c_uint64)]),
+        "rtg_scene_create": (C.c_int32, [_VP, P(SceneDesc), P(_VP)]),
+        "rtg_scene_release": (C.c_int32, [_VP]),
+        "rtg_scene_bytes": (C.c_uint64, [_VP]),
+        "rtg_accum_create": (C.c_int32, [_VP, C.c_int32, C.c_int32, P(_VP)]),
+        "rtg_accum_release": (C.c_int32, [_VP]),
+        "rtg_accum_clear": (C.c_int32, [_VP, _VP]),
+        "rtg_accum_read": (C.c_int32, [_VP, _VP, _VP, _VP]),
+        "rtg_accum_min_samples": (C.c_uint32, [_VP, _VP]),
+        "rtg_render_pass": (C.c_int32, [_VP, _VP, P(RenderSettings), C.c_uint32, C.c_uint32, _VP, _VP, P(Stats)]),
+        "rtg_resolve": (C.c_int32, [_VP, _VP, C.c_int32, C.c_double, _VP]),
+        "rtg_buffer_create": (C.c_int32, [_VP, C.c_uint64, P(_VP)]),
+        "rtg_buffer_release": (C.c_int32, [_VP]),
+        "rtg_buffer_write": (C.c_int32, [_VP, _VP, _VP, C.c_uint64]),
+        "rtg_buffer_read": (C.c_int32, [_VP, _VP, _VP, C.c_uint64]),
+        "rtg_trace": (C.c_int32, [_VP, _VP, _VP, C.c_uint64, P(TraceOpts), _VP, P(Stats)]),
+        "rtg_trace_host": (C.c_int32, [_VP, _VP, _VP, C.c_uint64, P(TraceOpts), _VP, P(Stats)]),
+        "rtg_synchronize": (C.c_int32, [_VP]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype, f.argtypes = res, args
+    _rtg = lib
+    return lib
+
+
+def rtg_symbols():
+    """Names declared in include/rtg.h (checked by the CPU tests)."""
+    return ["rtg_open", "rtg_close", "rtg_last_error", "rtg_abi_version", "rtg_device_info", "rtg_scene_create",
+            "rtg_scene_release", "rtg_scene_bytes", "rtg_accum_create", "rtg_accum_release", "rtg_accum_clear",
+            "rtg_accum_read", "rtg_accum_min_samples", "rtg_render_pass", "rtg_resolve", "rtg_buffer_create",
+            "rtg_buffer_release", "rtg_buffer_write", "rtg_buffer_read", "rtg_trace", "rtg_trace_host",
+            "rtg_synchronize"]
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Context:
+    """One HIP device + stream (rtg_open)."""
+
+    def __init__(self, device=0):
+        self.lib = load_rtg()
+        h = C.c_void_p()
+        self._check(self.lib.rtg_open(device, C.byref(h)), None)
+        self.h = h
+
+    def _check(self, st, h=-1):
+        if st != 0:
+            msg = self.lib.rtg_last_error(self.h if h == -1 else h)
+            raise RtgError(f"rtg status {st}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            self.lib.rtg_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_info(self):
+        name = C.create_string_buffer(256)
+        mp, mem = C.c_int(), C.c_uint64()
+        self._check(self.lib.rtg_device_info(self.h, name, 256, C.byref(mp), C.byref(mem)))
+        return {"name": name.value.decode(), "multiprocessors": mp.value, "hbm_bytes": mem.value}
+
+    def scene(self, desc: SceneDesc):
+        h = C.c_void_p()
+        self._check(self.lib.rtg_scene_create(self.h, C.byref(desc), C.byref(h)))
+        return DeviceScene(self, h)
+
+    def accum(self, w, h):
+        a = C.c_void_p()
+        self._check(self.lib.rtg_accum_create(self.h, w, h, C.byref(a)))
+        return Accum(self, a, w, h)
+
+    def render_pass(self, scene, settings: RenderSettings, s0, s1, acc, cancel=None):
+        st = Stats()
+        cp = C.cast(C.pointer(cancel), C.c_void_p) if cancel is not None else None
+        rc = self.lib.rtg_render_pass(self.h, scene.h, C.byref(settings), s0, s1, acc.h, cp, C.byref(st))
+        if rc == -4:
+            return st, True
+        self._check(rc)
+        return st, False
+
+    def trace(self, scene, rays: np.ndarray, mode=0, use_bvh=1, precision=PRECISION_F64):
+        rays = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 8)
+        hits = np.zeros(len(rays), dtype=HIT_DTYPE)
+        st = Stats()
+        opts = TraceOpts(mode, use_bvh, precision, 0)
+        self._check(self.lib.rtg_trace_host(self.h, scene.h, _ptr(rays), len(rays), C.byref(opts), _ptr(hits),
+                                            C.byref(st)))
+        return hits, st
+
+    def synchronize(self):
+        self._check(self.lib.rtg_synchronize(self.h))
+
+
+class DeviceScene:
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+
+    def nbytes(self):
+        return self.ctx.lib.rtg_scene_bytes(self.h)
+
+    def release(self):
+        if self.h:
+            self.ctx.lib.rtg_scene_release(self.h)
+            self.h = None
+
+
+class Accum:
+    def __init__(self, ctx, h, w, hh):
+        self.ctx, self.h, self.w, self.hgt = ctx, h, w, hh
+
+    def read(self):
+        s = np.zeros((self.hgt, self.w, 3), np.float64)
+        c = np.zeros((self.hgt, self.w), np.uint32)
+        self.ctx._check(self.ctx.lib.rtg_accum_read(self.ctx.h, self.h, _ptr(s), _ptr(c)))
+        return s, c
+
+    def clear(self):
+        self.ctx._check(self.ctx.lib.rtg_accum_clear(self.ctx.h, self.h))
+
+    def min_samples(self):
+        return self.ctx.lib.rtg_accum_min_samples(self.ctx.h, self.h)
+
+    def resolve(self, tone_map=0, exposure=1.0):
+        out = np.zeros((self.hgt, self.w, 3), np.uint8)
+        self.ctx._check(self.ctx.lib.rtg_resolve(self.ctx.h, self.h, tone_map, exposure, _ptr(out)))
+        return out
+
+    def release(self):
+        if self.h:
+            self.ctx.lib.rtg_accum_release(self.h)
+            self.h = None
+
+
+class HostArrays:
+    """Owns numpy copies of a compiled scene and exposes them as a SceneDesc."""
+
+    def __init__(self, prims, sampled, planes, materials, point_lights, lights, nodes, prim_indices, camera,
+                 background):
+        def arr(x, ctype):
+            n = len(x)
+            a = (ctype * max(n, 1))()
+            for i, v in enumerate(x):
+                a[i] = v
+            return a, n
+
+        self._keep = []
+        d = SceneDesc()
+        for field, ctype, data in (("prims", Primitive, prims), ("planes", Plane, planes),
+                                   ("materials", Material, materials), ("point_lights", PointLight, point_lights),
+                                   ("lights", LightRef, lights), ("nodes", BvhNode, nodes)):
+            a, n = data if isinstance(data, tuple) else arr(data, ctype)
+            self._keep.append(a)
+            setattr(d, field, C.cast(a, C.POINTER(ctype)))
+            setattr(d, "n_" + field, n)
+        s = (C.c_uint8 * max(len(sampled), 1))(*sampled)
+        pi = (C.c_uint32 * max(len(prim_indices), 1))(*prim_indices)
+        self._keep += [s, pi]
+        d.is_sampled_light = C.cast(s, C.POINTER(C.c_uint8))
+        d.prim_indices = C.cast(pi, C.POINTER(C.c_uint32))
+        d.n_prim_indices = len(prim_indices)
+        d.camera = camera
+        d.background = background
+        self.desc = d
