@@ -1,0 +1,717 @@
+// rtg_api.hip — the C ABI of include/rtg.h: device contexts, scene upload
+// (flattening the host-built BVH into the FP32 two-child layout), device
+// accumulation buffers, the render-pass driver and batch ray queries.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rtg_launch.hpp"
+
+using namespace rtg;
+
+namespace {
+thread_local std::string g_error;
+
+struct Status {
+  rtg_status code;
+  std::string msg;
+};
+}  // namespace
+
+struct rtg_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  unsigned long long* stats = nullptr;
+  // Path state, grown on demand per precision.
+  void* path_mem[2] = {nullptr, nullptr};
+  uint64_t path_bytes[2] = {0, 0};
+  PathBuffers<float> pb32{};
+  PathBuffers<double> pb64{};
+  uint32_t cap[2] = {0, 0};
+};
+
+struct rtg_scene {
+  rtg_context* ctx = nullptr;
+  DevScene dev{};
+  std::vector<void*> allocs;
+  uint64_t bytes = 0;
+  uint32_t stack = 0;  // BVH depth + 1
+};
+
+struct rtg_accum {
+  rtg_context* ctx = nullptr;
+  int32_t w = 0, h = 0;
+  double* sum = nullptr;
+  uint32_t* count = nullptr;
+};
+
+struct rtg_buffer {
+  rtg_context* ctx = nullptr;
+  void* ptr = nullptr;
+  uint64_t bytes = 0;
+};
+
+namespace {
+
+rtg_status fail(rtg_context* ctx, rtg_status code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  g_error = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                              \
+  do {                                                                                                  \
+    const hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess)                                                                               \
+      return fail(ctx, e_ == hipErrorOutOfMemory ? RTG_ERR_OUT_OF_MEMORY : RTG_ERR_HIP,                 \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                                   \
+  } while (0)
+
+// Outward rounding of the double boxes to float: the float box always
+// contains the double box.
+float round_down(double v) {
+  float f = (float)v;
+  if ((double)f > v) f = std::nextafter(f, -INFINITY);
+  return f;
+}
+float round_up(double v) {
+  float f = (float)v;
+  if ((double)f < v) f = std::nextafter(f, INFINITY);
+  return f;
+}
+float bits_f(uint32_t u) {
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+float4 f4(float x, float y, float z, float w) { return float4{x, y, z, w}; }
+
+struct Upload {
+  rtg_scene* s;
+  rtg_context* ctx;
+  template <class T>
+  rtg_status put(const T* host, size_t n, const T** dev) {
+    if (n == 0) {
+      *dev = nullptr;
+      return RTG_OK;
+    }
+    void* p = nullptr;
+    HIP_TRY(ctx, hipMalloc(&p, n * sizeof(T)));
+    s->allocs.push_back(p);
+    s->bytes += n * sizeof(T);
+    HIP_TRY(ctx, hipMemcpy(p, host, n * sizeof(T), hipMemcpyHostToDevice));
+    *dev = static_cast<const T*>(p);
+    return RTG_OK;
+  }
+};
+
+rtg_status validate_desc(rtg_context* ctx, const rtg_scene_desc* d) {
+  auto bad = [&](const std::string& m) { return fail(ctx, RTG_ERR_INVALID_ARG, "scene: " + m); };
+  if ((d->n_prims && !d->prims) || (d->n_planes && !d->planes) || (d->n_materials && !d->materials) ||
+      (d->n_lights && !d->lights) || (d->n_nodes && !d->nodes) || (d->n_prim_indices && !d->prim_indices) ||
+      (d->n_point_lights && !d->point_lights) || (d->n_prims && !d->is_sampled_light))
+    return bad("null array with non-zero count");
+  for (uint32_t i = 0; i < d->n_prims; ++i) {
+    if (d->prims[i].material >= d->n_materials) return bad("primitive material out of range");
+    if (d->prims[i].kind > RTG_PRIM_TRIANGLE) return bad("unknown primitive kind");
+  }
+  for (uint32_t i = 0; i < d->n_planes; ++i)
+    if (d->planes[i].material >= d->n_materials) return bad("plane material out of range");
+  for (uint32_t i = 0; i < d->n_lights; ++i) {
+    const rtg_light_ref& l = d->lights[i];
+    if (l.kind == RTG_LIGHT_POINT ? l.index >= d->n_point_lights : l.index >= d->n_prims)
+      return bad("light index out of range");
+  }
+  for (uint32_t i = 0; i < d->n_prim_indices; ++i)
+    if (d->prim_indices[i] >= d->n_prims) return bad("prim_indices out of range");
+  for (uint32_t i = 0; i < d->n_nodes; ++i) {
+    const rtg_bvh_node& n = d->nodes[i];
+    if (n.count > 0) {
+      if ((uint64_t)n.offset + n.count > d->n_prim_indices) return bad("leaf range out of prim_indices");
+    } else {
+      if (n.offset <= i || n.offset >= d->n_nodes || i + 1 >= d->n_nodes) return bad("interior child out of range");
+      if (n.axis > 2) return bad("split axis out of range");
+    }
+  }
+  return RTG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtg_abi_version(void) { return RTG_ABI_VERSION; }
+
+const char* rtg_last_error(const rtg_context* ctx) { return ctx ? ctx->err.c_str() : g_error.c_str(); }
+
+rtg_status rtg_open(int device, rtg_context** out) {
+  if (!out) return fail(nullptr, RTG_ERR_INVALID_ARG, "rtg_open: out is NULL");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(nullptr, RTG_ERR_NO_DEVICE, "no HIP device");
+  if (device < 0 || device >= n) return fail(nullptr, RTG_ERR_INVALID_ARG, "rtg_open: device out of range");
+  auto* c = new rtg_context;
+  c->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipMalloc(&c->stats, sizeof(unsigned long long) * kStCount);
+  if (e != hipSuccess) {
+    const std::string m = std::string("rtg_open: ") + hipGetErrorString(e);
+    delete c;
+    return fail(nullptr, RTG_ERR_HIP, m);
+  }
+  *out = c;
+  return RTG_OK;
+}
+
+rtg_status rtg_close(rtg_context* ctx) {
+  if (!ctx) return RTG_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (void* p : ctx->path_mem)
+    if (p) hipFree(p);
+  if (ctx->stats) hipFree(ctx->stats);
+  if (ctx->ev0) hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return RTG_OK;
+}
+
+rtg_status rtg_device_info(rtg_context* ctx, char* name, size_t name_len, int* mps, uint64_t* hbm) {
+  if (!ctx) return fail(nullptr, RTG_ERR_INVALID_ARG, "rtg_device_info: NULL context");
+  hipDeviceProp_t p;
+  HIP_TRY(ctx, hipGetDeviceProperties(&p, ctx->device));
+  if (name && name_len) {
+    std::strncpy(name, p.gcnArchName, name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (mps) *mps = p.multiProcessorCount;
+  if (hbm) *hbm = p.totalGlobalMem;
+  return RTG_OK;
+}
+
+rtg_status rtg_synchronize(rtg_context* ctx) {
+  if (!ctx) return fail(nullptr, RTG_ERR_INVALID_ARG, "NULL context");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return RTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* d, rtg_scene** out) {
+  if (!ctx || !d || !out) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_scene_create: NULL argument");
+  if (rtg_status st = validate_desc(ctx, d)) return st;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  auto* s = new rtg_scene;
+  s->ctx = ctx;
+  Upload up{s, ctx};
+  DevScene& S = s->dev;
+  auto cleanup = [&](rtg_status st) {
+    for (void* p : s->allocs) hipFree(p);
+    delete s;
+    return st;
+  };
+  const uint32_t np = d->n_prims;
+
+  // --- BVH depth and the FP32 two-child node layout ---
+  std::vector<int32_t> n2_index(d->n_nodes, -1);
+  uint32_t n_interior = 0, max_depth = 0;
+  {
+    std::vector<std::pair<uint32_t, uint32_t>> st;  // (node, depth)
+    if (d->n_nodes) st.push_back({0, 0});
+    while (!st.empty()) {
+      auto [i, dep] = st.back();
+      st.pop_back();
+      max_depth = std::max(max_depth, dep);
+      const rtg_bvh_node& n = d->nodes[i];
+      if (n.count == 0) {
+        st.push_back({n.offset, dep + 1});
+        st.push_back({i + 1, dep + 1});
+      } else if (n.count > kMaxLeafCount) {
+        return cleanup(fail(ctx, RTG_ERR_UNSUPPORTED, "scene: BVH leaf with more than 64 primitives"));
+      }
+    }
+    for (uint32_t i = 0; i < d->n_nodes; ++i)
+      if (d->nodes[i].count == 0) n2_index[i] = (int32_t)n_interior++;
+  }
+  if (np > kMaxSlots) return cleanup(fail(ctx, RTG_ERR_UNSUPPORTED, "scene: more than 2^25 primitives"));
+  s->stack = max_depth + 1;
+  if (s->stack > 64) return cleanup(fail(ctx, RTG_ERR_UNSUPPORTED, "scene: BVH deeper than 63 levels"));
+
+  auto child_ref = [&](uint32_t c) -> uint32_t {
+    const rtg_bvh_node& n = d->nodes[c];
+    return n.count > 0 ? leaf_ref(n.offset, n.count) : (uint32_t)n2_index[c];
+  };
+  std::vector<float4> node2((size_t)n_interior * kNode2Stride);
+  for (uint32_t i = 0; i < d->n_nodes; ++i) {
+    const rtg_bvh_node& n = d->nodes[i];
+    if (n.count > 0) continue;
+    const rtg_bvh_node& L = d->nodes[i + 1];
+    const rtg_bvh_node& R = d->nodes[n.offset];
+    float4* o = &node2[(size_t)n2_index[i] * kNode2Stride];
+    o[0] = f4(round_down(L.lo.x), round_down(L.lo.y), round_down(L.lo.z), round_up(L.hi.x));
+    o[1] = f4(round_up(L.hi.y), round_up(L.hi.z), round_down(R.lo.x), round_down(R.lo.y));
+    o[2] = f4(round_down(R.lo.z), round_up(R.hi.x), round_up(R.hi.y), round_up(R.hi.z));
+    o[3] = f4(bits_f(child_ref(i + 1)), bits_f(child_ref(n.offset)), bits_f(n.axis), 0.0f);
+  }
+
+  // --- slots: leaf order first, then primitives outside the BVH ---
+  std::vector<uint32_t> prim_slot(np, 0xFFFFFFFFu);
+  std::vector<uint32_t> slot_prim;
+  slot_prim.reserve(np);
+  for (uint32_t k = 0; k < d->n_prim_indices; ++k) {
+    prim_slot[d->prim_indices[k]] = (uint32_t)slot_prim.size();
+    slot_prim.push_back(d->prim_indices[k]);
+  }
+  for (uint32_t i = 0; i < np; ++i)
+    if (prim_slot[i] == 0xFFFFFFFFu) {
+      prim_slot[i] = (uint32_t)slot_prim.size();
+      slot_prim.push_back(i);
+    }
+  std::vector<float4> slot32((size_t)slot_prim.size() * kSlot32Stride);
+  std::vector<float4> shade32((size_t)np * kShade32Stride);
+  for (uint32_t k = 0; k < slot_prim.size(); ++k) {
+    const uint32_t i = slot_prim[k];
+    const rtg_primitive& p = d->prims[i];
+    float4* o = &slot32[(size_t)k * kSlot32Stride];
+    if (p.kind == RTG_PRIM_SPHERE) {
+      const double r = p.v1.x;
+      o[0] = f4((float)p.v0.x, (float)p.v0.y, (float)p.v0.z, bits_f(i | kSlotSphere));
+      o[1] = f4((float)r, (float)(r * r), 0.0f, 0.0f);
+      o[2] = f4(0, 0, 0, 0);
+    } else {
+      o[0] = f4((float)p.v0.x, (float)p.v0.y, (float)p.v0.z, bits_f(i));
+      o[1] = f4((float)(p.v1.x - p.v0.x), (float)(p.v1.y - p.v0.y), (float)(p.v1.z - p.v0.z), 0.0f);
+      o[2] = f4((float)(p.v2.x - p.v0.x), (float)(p.v2.y - p.v0.y), (float)(p.v2.z - p.v0.z), 0.0f);
+    }
+  }
+  for (uint32_t i = 0; i < np; ++i) {
+    const rtg_primitive& p = d->prims[i];
+    float4* o = &shade32[(size_t)i * kShade32Stride];
+    if (p.kind == RTG_PRIM_SPHERE) {
+      o[0] = f4((float)p.v0.x, (float)p.v0.y, (float)p.v0.z, bits_f(p.material | kShadeSphere));
+      o[1] = f4((float)p.v1.x, (float)(1.0 / p.v1.x), 0, 0);
+      o[2] = o[3] = f4(0, 0, 0, 0);
+    } else {
+      const double e1[3] = {p.v1.x - p.v0.x, p.v1.y - p.v0.y, p.v1.z - p.v0.z};
+      const double e2[3] = {p.v2.x - p.v0.x, p.v2.y - p.v0.y, p.v2.z - p.v0.z};
+      double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      const double l2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+      const double inv = (l2 > 1e-18 && std::isfinite(l2)) ? 1.0 / std::sqrt(l2) : 0.0;
+      const uint32_t w = p.material | (p.has_vertex_normals ? kShadeSmooth : 0u);
+      o[0] = f4((float)(n[0] * inv), (float)(n[1] * inv), (float)(n[2] * inv), bits_f(w));
+      o[1] = f4((float)p.n0.x, (float)p.n0.y, (float)p.n0.z, (float)p.n1.x);
+      o[2] = f4((float)p.n1.y, (float)p.n1.z, (float)p.n2.x, (float)p.n2.y);
+      o[3] = f4((float)p.n2.z, 0, 0, 0);
+    }
+  }
+  std::vector<float4> mat32((size_t)d->n_materials * 2);
+  for (uint32_t i = 0; i < d->n_materials; ++i) {
+    const rtg_material& m = d->materials[i];
+    mat32[2 * i] = f4((float)m.c0.x, (float)m.c0.y, (float)m.c0.z, bits_f(m.kind | (m.two_sided ? 256u : 0u)));
+    mat32[2 * i + 1] = f4((float)m.c1.x, (float)m.c1.y, (float)m.c1.z, (float)m.p0);
+  }
+  std::vector<float4> plane32((size_t)d->n_planes * 2);
+  for (uint32_t i = 0; i < d->n_planes; ++i) {
+    const rtg_plane& p = d->planes[i];
+    plane32[2 * i] = f4((float)p.point.x, (float)p.point.y, (float)p.point.z, bits_f(p.material));
+    plane32[2 * i + 1] = f4((float)p.normal.x, (float)p.normal.y, (float)p.normal.z, 0);
+  }
+  std::vector<float4> light32((size_t)d->n_lights * kLight32Stride, f4(0, 0, 0, 0));
+  for (uint32_t i = 0; i < d->n_lights; ++i) {
+    const rtg_light_ref& l = d->lights[i];
+    float4* o = &light32[(size_t)i * kLight32Stride];
+    if (l.kind == RTG_LIGHT_POINT) {
+      const rtg_point_light& pl = d->point_lights[l.index];
+      o[0] = f4((float)pl.position.x, (float)pl.position.y, (float)pl.position.z, bits_f(kL32Point));
+      o[1] = f4((float)pl.intensity.x, (float)pl.intensity.y, (float)pl.intensity.z, 0);
+      continue;
+    }
+    const rtg_primitive& p = d->prims[l.index];
+    const rtg_material& m = d->materials[p.material];
+    const double em[3] = {m.c0.x * m.p0, m.c0.y * m.p0, m.c0.z * m.p0};
+    const float ts = m.two_sided ? 1.0f : 0.0f;
+    if (p.kind == RTG_PRIM_SPHERE) {
+      const double r = p.v1.x;
+      o[0] = f4((float)p.v0.x, (float)p.v0.y, (float)p.v0.z, bits_f(kL32Sphere));
+      o[1] = f4((float)r, (float)(r * r), (float)(4 * M_PI * r * r), ts);
+      o[2] = f4((float)em[0], (float)em[1], (float)em[2], 0);
+    } else {
+      const double e1[3] = {p.v1.x - p.v0.x, p.v1.y - p.v0.y, p.v1.z - p.v0.z};
+      const double e2[3] = {p.v2.x - p.v0.x, p.v2.y - p.v0.y, p.v2.z - p.v0.z};
+      const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      const double inv = len > 1e-9 ? 1.0 / len : 0.0;
+      o[0] = f4((float)p.v0.x, (float)p.v0.y, (float)p.v0.z, bits_f(kL32Triangle));
+      o[1] = f4((float)e1[0], (float)e1[1], (float)e1[2], (float)(0.5 * len));
+      o[2] = f4((float)e2[0], (float)e2[1], (float)e2[2], ts);
+      o[3] = f4((float)em[0], (float)em[1], (float)em[2], 0);
+      o[4] = f4((float)(n[0] * inv), (float)(n[1] * inv), (float)(n[2] * inv), 0);
+    }
+  }
+
+  // --- uploads ---
+  rtg_status st = RTG_OK;
+#define PUT(host, n, dst)                                  \
+  if ((st = up.put((host), (n), &(dst))) != RTG_OK) return cleanup(st)
+  PUT(node2.data(), node2.size(), S.node2);
+  PUT(slot32.data(), slot32.size(), S.slot32);
+  PUT(shade32.data(), shade32.size(), S.shade32);
+  PUT(prim_slot.data(), prim_slot.size(), S.prim_slot);
+  PUT(mat32.data(), mat32.size(), S.mat32);
+  PUT(plane32.data(), plane32.size(), S.plane32);
+  PUT(light32.data(), light32.size(), S.light32);
+  PUT(d->nodes, (size_t)d->n_nodes, S.nodes);
+  PUT(d->prim_indices, (size_t)d->n_prim_indices, S.prim_indices);
+  PUT(d->prims, (size_t)np, S.prims);
+  PUT(d->is_sampled_light, (size_t)np, S.sampled);
+  PUT(d->materials, (size_t)d->n_materials, S.materials);
+  PUT(d->planes, (size_t)d->n_planes, S.planes);
+  PUT(d->lights, (size_t)d->n_lights, S.lights);
+  PUT(d->point_lights, (size_t)d->n_point_lights, S.point_lights);
+#undef PUT
+  S.has_bvh = d->n_nodes > 0;
+  S.n_nodes = d->n_nodes;
+  S.n_prims = np;
+  S.n_slots = (uint32_t)slot_prim.size();
+  S.n_planes = d->n_planes;
+  S.n_lights = d->n_lights;
+  S.n_materials = d->n_materials;
+  S.stack_depth = s->stack;
+  if (d->n_nodes) {
+    const rtg_bvh_node& r = d->nodes[0];
+    S.root_ref = r.count > 0 ? leaf_ref(r.offset, r.count) : 0u;
+    S.root_lo[0] = round_down(r.lo.x); S.root_lo[1] = round_down(r.lo.y); S.root_lo[2] = round_down(r.lo.z);
+    S.root_hi[0] = round_up(r.hi.x); S.root_hi[1] = round_up(r.hi.y); S.root_hi[2] = round_up(r.hi.z);
+  }
+  S.cam = d->camera;
+  S.bg = d->background;
+  const rtg_camera& c = d->camera;
+  const double cam[19] = {c.origin.x, c.origin.y, c.origin.z, c.corner.x - c.origin.x, c.corner.y - c.origin.y,
+                          c.corner.z - c.origin.z, c.du.x, c.du.y, c.du.z, c.dv.x, c.dv.y, c.dv.z,
+                          c.u.x, c.u.y, c.u.z, c.v.x, c.v.y, c.v.z, c.lens_radius};
+  for (int i = 0; i < 19; ++i) S.cam32[i] = (float)cam[i];
+  const rtg_background& b = d->background;
+  const double bg[7] = {b.top.x, b.top.y, b.top.z, b.bottom.x, b.bottom.y, b.bottom.z, b.intensity};
+  for (int i = 0; i < 7; ++i) S.bg32[i] = (float)bg[i];
+  S.bg_solid = b.kind == RTG_BG_SOLID;
+  *out = s;
+  return RTG_OK;
+}
+
+rtg_status rtg_scene_release(rtg_scene* s) {
+  if (!s) return RTG_OK;
+  hipSetDevice(s->ctx->device);
+  hipStreamSynchronize(s->ctx->stream);
+  for (void* p : s->allocs) hipFree(p);
+  delete s;
+  return RTG_OK;
+}
+
+uint64_t rtg_scene_bytes(const rtg_scene* s) { return s ? s->bytes : 0; }
+
+// ---------------------------------------------------------------------------
+rtg_status rtg_accum_create(rtg_context* ctx, int32_t w, int32_t h, rtg_accum** out) {
+  if (!ctx || !out || w < 1 || h < 1 || w > 16384 || h > 16384)
+    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_accum_create: bad arguments");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  auto* a = new rtg_accum;
+  a->ctx = ctx;
+  a->w = w;
+  a->h = h;
+  const size_t n = (size_t)w * h;
+  hipError_t e = hipMalloc(&a->sum, n * 3 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&a->count, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemsetAsync(a->sum, 0, n * 3 * sizeof(double), ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(a->count, 0, n * sizeof(uint32_t), ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    if (a->sum) hipFree(a->sum);
+    if (a->count) hipFree(a->count);
+    delete a;
+    return fail(ctx, e == hipErrorOutOfMemory ? RTG_ERR_OUT_OF_MEMORY : RTG_ERR_HIP,
+                std::string("rtg_accum_create: ") + hipGetErrorString(e));
+  }
+  *out = a;
+  return RTG_OK;
+}
+
+rtg_status rtg_accum_release(rtg_accum* a) {
+  if (!a) return RTG_OK;
+  hipSetDevice(a->ctx->device);
+  hipStreamSynchronize(a->ctx->stream);
+  hipFree(a->sum);
+  hipFree(a->count);
+  delete a;
+  return RTG_OK;
+}
+
+rtg_status rtg_accum_clear(rtg_context* ctx, rtg_accum* a) {
+  if (!ctx || !a) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_accum_clear: NULL argument");
+  const size_t n = (size_t)a->w * a->h;
+  HIP_TRY(ctx, hipMemsetAsync(a->sum, 0, n * 3 * sizeof(double), ctx->stream));
+  HIP_TRY(ctx, hipMemsetAsync(a->count, 0, n * sizeof(uint32_t), ctx->stream));
+  return RTG_OK;
+}
+
+rtg_status rtg_accum_read(rtg_context* ctx, const rtg_accum* a, double* sum, uint32_t* count) {
+  if (!ctx || !a) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_accum_read: NULL argument");
+  const size_t n = (size_t)a->w * a->h;
+  if (sum) HIP_TRY(ctx, hipMemcpyAsync(sum, a->sum, n * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (count) HIP_TRY(ctx, hipMemcpyAsync(count, a->count, n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return RTG_OK;
+}
+
+uint32_t rtg_accum_min_samples(rtg_context* ctx, const rtg_accum* a) {
+  if (!ctx || !a) return 0;
+  std::vector<uint32_t> c((size_t)a->w * a->h);
+  if (rtg_accum_read(ctx, a, nullptr, c.data()) != RTG_OK) return 0;
+  return *std::min_element(c.begin(), c.end());
+}
+
+rtg_status rtg_resolve(rtg_context* ctx, const rtg_accum* a, int32_t tone_map, double exposure, uint8_t* rgb) {
+  if (!ctx || !a || !rgb || tone_map < 0 || tone_map > 2)
+    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_resolve: bad arguments");
+  const size_t n = (size_t)a->w * a->h * 3;
+  uint8_t* dev = nullptr;
+  HIP_TRY(ctx, hipMallocAsync((void**)&dev, n, ctx->stream));
+  run_resolve(a->sum, a->count, a->w, a->h, tone_map, exposure, dev, ctx->stream);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipMemcpyAsync(rgb, dev, n, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipFreeAsync(dev, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return RTG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+namespace {
+
+template <class T>
+rtg_status ensure_paths(rtg_context* ctx, int which, uint32_t cap, PathBuffers<T>& B) {
+  if (ctx->cap[which] >= cap) return RTG_OK;
+  if (ctx->path_mem[which]) {
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipFree(ctx->path_mem[which]));
+    ctx->path_mem[which] = nullptr;
+    ctx->cap[which] = 0;
+  }
+  const uint64_t c = cap;
+  auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
+  const uint64_t tbytes = al(c * sizeof(T));
+  // 6 ray + 3 beta + 3 L + 3 hit + 10 shadow arrays of T; 2 RNG u64; 5 u32
+  // (2 queues, hprim, hplane, sq_pid); spec; counts.
+  const uint64_t bytes = tbytes * 25 + al(c * 8) * 2 + al(c * 4) * 5 + al(c) + al(sizeof(uint32_t) * 2 * kMaxBounce);
+  void* mem = nullptr;
+  HIP_TRY(ctx, hipMalloc(&mem, bytes));
+  char* p = static_cast<char*>(mem);
+  auto take = [&](uint64_t b) {
+    char* r = p;
+    p += al(b);
+    return r;
+  };
+  T** tarr[] = {&B.ox, &B.oy, &B.oz, &B.dx, &B.dy, &B.dz, &B.bx, &B.by, &B.bz, &B.lx, &B.ly, &B.lz, &B.ht,
+                &B.hb1, &B.hb2, &B.sox, &B.soy, &B.soz, &B.sdx, &B.sdy, &B.sdz, &B.stm, &B.scx, &B.scy, &B.scz};
+  for (T** a : tarr) *a = reinterpret_cast<T*>(take(c * sizeof(T)));
+  B.rs = reinterpret_cast<uint64_t*>(take(c * 8));
+  B.ri = reinterpret_cast<uint64_t*>(take(c * 8));
+  B.q[0] = reinterpret_cast<uint32_t*>(take(c * 4));
+  B.q[1] = reinterpret_cast<uint32_t*>(take(c * 4));
+  B.hprim = reinterpret_cast<uint32_t*>(take(c * 4));
+  B.hplane = reinterpret_cast<int32_t*>(take(c * 4));
+  B.sq_pid = reinterpret_cast<uint32_t*>(take(c * 4));
+  B.spec = reinterpret_cast<uint8_t*>(take(c));
+  B.counts = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 2 * kMaxBounce));
+  B.stats = ctx->stats;
+  B.capacity = cap;
+  ctx->path_mem[which] = mem;
+  ctx->path_bytes[which] = bytes;
+  ctx->cap[which] = cap;
+  return RTG_OK;
+}
+
+uint32_t chunk_paths_default() {
+  if (const char* e = std::getenv("RTG_CHUNK_PATHS")) {
+    const long v = std::atol(e);
+    if (v >= 1024 && v <= (1L << 28)) return (uint32_t)v;
+  }
+  return 1u << 22;
+}
+
+void read_stats(rtg_context* ctx, rtg_stats* out, float ms) {
+  unsigned long long h[kStCount] = {0};
+  hipMemcpy(h, ctx->stats, sizeof h, hipMemcpyDeviceToHost);
+  if (!out) return;
+  out->camera_samples = h[kStCamera];
+  out->rays = h[kStRays];
+  out->shadow_rays = h[kStShadow];
+  out->rr_terminated = h[kStRR];
+  out->invalid_samples = h[kStInvalid];
+  out->node_visits = h[kStVisits];
+  out->prim_tests = h[kStTests];
+  out->elapsed_ms = ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+rtg_status rtg_render_pass(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* cfg, uint32_t s0,
+                           uint32_t s1, rtg_accum* acc, const volatile int32_t* cancel, rtg_stats* stats) {
+  if (!ctx || !scene || !cfg || !acc) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_pass: NULL argument");
+  if (cfg->width != acc->w || cfg->height != acc->h)
+    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_pass: settings size differs from the accumulation buffer");
+  if (s1 <= s0) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_pass: empty sample range");
+  if (cfg->max_depth < 1 || cfg->max_depth >= kMaxBounce)
+    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_pass: max_depth must be in [1, 1023]");
+  if (cfg->integrator < 0 || cfg->integrator > 2 || cfg->precision < 0 || cfg->precision > 1)
+    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_pass: bad integrator or precision");
+  if (cfg->integrator == RTG_INTEGRATOR_DIRECT && cfg->max_depth > 31)
+    return fail(ctx, RTG_ERR_UNSUPPORTED, "rtg_render_pass: the direct integrator supports max_depth <= 31");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const bool f64 = cfg->precision == RTG_PRECISION_F64;
+  const uint32_t cap = chunk_paths_default();
+  if (f64) {
+    if (rtg_status st = ensure_paths(ctx, 1, cap, ctx->pb64)) return st;
+  } else {
+    if (rtg_status st = ensure_paths(ctx, 0, cap, ctx->pb32)) return st;
+  }
+  HIP_TRY(ctx, hipMemsetAsync(ctx->stats, 0, sizeof(unsigned long long) * kStCount, ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  const uint32_t k_total = s1 - s0;
+  const uint32_t n_pix = (uint32_t)acc->w * (uint32_t)acc->h;
+  bool cancelled = false;
+  // Sample sub-ranges outermost keep every pixel's additions in sample order.
+  for (uint32_t sa = s0; sa < s1 && !cancelled;) {
+    const uint32_t k = std::min(k_total - (sa - s0), cap);
+    const uint32_t pix_per_chunk = std::max(1u, cap / k);
+    for (uint32_t p0 = 0; p0 < n_pix; p0 += pix_per_chunk) {
+      if (cancel && *cancel) {
+        cancelled = true;
+        break;
+      }
+      ChunkParams P{};
+      P.width = (uint32_t)acc->w;
+      P.height = (uint32_t)acc->h;
+      P.p0 = p0;
+      P.k = k;
+      P.s0 = sa;
+      P.n_pix = std::min(pix_per_chunk, n_pix - p0);
+      P.n_paths = P.n_pix * k;
+      P.max_depth = cfg->max_depth;
+      P.rr_min_depth = cfg->rr_min_depth;
+      P.integrator = cfg->integrator;
+      P.light_sampling = cfg->light_sampling;
+      P.use_bvh = cfg->use_bvh;
+      P.seed = cfg->seed;
+      P.acc_sum = acc->sum;
+      P.acc_count = acc->count;
+      if (f64) run_chunk(scene->dev, ctx->pb64, P, scene->stack, ctx->stream);
+      else run_chunk(scene->dev, ctx->pb32, P, scene->stack, ctx->stream);
+      HIP_TRY(ctx, hipGetLastError());
+      if (cancel) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // poll between chunks
+    }
+    sa += k;
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  read_stats(ctx, stats, ms);
+  if (cancelled) return fail(ctx, RTG_ERR_CANCELLED, "rtg_render_pass: cancelled");
+  return RTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+rtg_status rtg_buffer_create(rtg_context* ctx, uint64_t bytes, rtg_buffer** out) {
+  if (!ctx || !out || bytes == 0) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_buffer_create: bad arguments");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  void* p = nullptr;
+  HIP_TRY(ctx, hipMalloc(&p, bytes));
+  auto* b = new rtg_buffer;
+  b->ctx = ctx;
+  b->ptr = p;
+  b->bytes = bytes;
+  *out = b;
+  return RTG_OK;
+}
+
+rtg_status rtg_buffer_release(rtg_buffer* b) {
+  if (!b) return RTG_OK;
+  hipSetDevice(b->ctx->device);
+  hipStreamSynchronize(b->ctx->stream);
+  hipFree(b->ptr);
+  delete b;
+  return RTG_OK;
+}
+
+rtg_status rtg_buffer_write(rtg_context* ctx, rtg_buffer* b, const void* src, uint64_t bytes) {
+  if (!ctx || !b || !src || bytes > b->bytes) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_buffer_write: bad arguments");
+  HIP_TRY(ctx, hipMemcpyAsync(b->ptr, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return RTG_OK;
+}
+
+rtg_status rtg_buffer_read(rtg_context* ctx, const rtg_buffer* b, void* dst, uint64_t bytes) {
+  if (!ctx || !b || !dst || bytes > b->bytes) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_buffer_read: bad arguments");
+  HIP_TRY(ctx, hipMemcpyAsync(dst, b->ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return RTG_OK;
+}
+
+rtg_status rtg_trace(rtg_context* ctx, const rtg_scene* scene, const rtg_buffer* rays, uint64_t n,
+                     const rtg_trace_opts* opts, rtg_buffer* hits, rtg_stats* stats) {
+  if (!ctx || !scene || !rays || !hits || !opts) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_trace: NULL argument");
+  if (n * 64 > rays->bytes || n * sizeof(rtg_hit) > hits->bytes)
+    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_trace: buffers too small for n rays");
+  if (opts->mode < 0 || opts->mode > 1 || opts->precision < 0 || opts->precision > 1)
+    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_trace: bad mode or precision");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMemsetAsync(ctx->stats, 0, sizeof(unsigned long long) * kStCount, ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (n) {
+    if (opts->precision == RTG_PRECISION_F64)
+      run_trace64(scene->dev, static_cast<const double*>(rays->ptr), n, opts->mode, opts->use_bvh,
+                  static_cast<rtg_hit*>(hits->ptr), ctx->stats, ctx->stream);
+    else
+      run_trace32(scene->dev, static_cast<const double*>(rays->ptr), n, opts->mode, opts->use_bvh,
+                  static_cast<rtg_hit*>(hits->ptr), ctx->stats, scene->stack, ctx->stream);
+    HIP_TRY(ctx, hipGetLastError());
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  read_stats(ctx, stats, ms);
+  return RTG_OK;
+}
+
+rtg_status rtg_trace_host(rtg_context* ctx, const rtg_scene* scene, const double* rays, uint64_t n,
+                          const rtg_trace_opts* opts, rtg_hit* hits, rtg_stats* stats) {
+  if (!ctx || !scene || (n && (!rays || !hits))) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_trace_host: NULL argument");
+  if (n == 0) {
+    if (stats) *stats = rtg_stats{};
+    return RTG_OK;
+  }
+  rtg_buffer *rb = nullptr, *hb = nullptr;
+  rtg_status st = rtg_buffer_create(ctx, n * 64, &rb);
+  if (st == RTG_OK) st = rtg_buffer_create(ctx, n * sizeof(rtg_hit), &hb);
+  if (st == RTG_OK) st = rtg_buffer_write(ctx, rb, rays, n * 64);
+  if (st == RTG_OK) st = rtg_trace(ctx, scene, rb, n, opts, hb, stats);
+  if (st == RTG_OK) st = rtg_buffer_read(ctx, hb, hits, n * sizeof(rtg_hit));
+  rtg_buffer_release(rb);
+  rtg_buffer_release(hb);
+  return st;
+}
+
+}  // extern "C"

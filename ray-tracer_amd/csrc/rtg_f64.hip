@@ -1,0 +1,73 @@
+// rtg_f64.hip — FP64 instantiation of the path-tracing kernels (the exact
+// mode: the reference's algorithm, layouts and PCG32 stream in double).
+// Built with -ffp-contract=off so no multiply-add is fused: every operation
+// rounds as in the reference's x86-64 build.
+#include "rtg_launch.hpp"
+
+namespace rtg {
+
+namespace {
+inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+// tonemap.cpp:14-53
+__device__ double clamp01(double v) { return v < 0 ? 0 : (v > 1 ? 1 : v); }
+__device__ double aces(double x) {
+  const double a = 2.51, b = 0.03, c = 2.43, d = 0.59, e = 0.14;
+  const double v = (x * (a * x + b)) / (x * (c * x + d) + e);
+  return v < 0 ? 0 : (v > 1 ? 1 : v);
+}
+__device__ uint8_t srgb8(double v) {
+  v = clamp01(v);
+  const double s = v <= 0.0031308 ? 12.92 * v : 1.055 * pow(v, 1.0 / 2.4) - 0.055;
+  return (uint8_t)llround(clamp01(s) * 255.0);
+}
+
+__global__ void k_resolve(const double* sum, const uint32_t* count, int n, int op, double exposure, uint8_t* rgb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double c[3] = {0, 0, 0};
+  if (count[i]) {
+    const double inv = 1.0 / (double)count[i];
+    for (int k = 0; k < 3; ++k) c[k] = sum[3 * (size_t)i + k] * inv;
+  }
+  for (int k = 0; k < 3; ++k) c[k] = c[k] * exposure;
+  if (!(isfinite(c[0]) && isfinite(c[1]) && isfinite(c[2]))) c[0] = c[1] = c[2] = 0;
+  for (int k = 0; k < 3; ++k) {
+    double v = c[k];
+    if (op == 0) v = clamp01(v);
+    else if (op == 1) v = v / (1 + v);
+    else v = aces(v);
+    rgb[3 * (size_t)i + k] = srgb8(v);
+  }
+}
+}  // namespace
+
+void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t, hipStream_t s) {
+  const uint32_t g = grid_of(P.n_paths);
+  if (P.integrator != RTG_INTEGRATOR_PATH) {
+    hipLaunchKernelGGL((k_direct<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P);
+  } else {
+    hipLaunchKernelGGL(k_generate<double>, dim3(g), dim3(kBlock), 0, s, S, B, P);
+    const bool nee = P.light_sampling && S.n_lights > 0;
+    for (int b = 0; b < P.max_depth; ++b) {
+      hipLaunchKernelGGL((k_extend<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      hipLaunchKernelGGL(k_shade<double>, dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      if (nee) hipLaunchKernelGGL((k_shadow<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+    }
+  }
+  hipLaunchKernelGGL(k_accumulate<double>, dim3(grid_of(P.n_pix)), dim3(kBlock), 0, s, B, P);
+}
+
+void run_trace64(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,
+                 unsigned long long* stats, hipStream_t s) {
+  hipLaunchKernelGGL((k_trace<double, 0>), dim3(grid_of(n)), dim3(kBlock), 0, s, S, rays, n, mode, use_bvh, hits,
+                     stats);
+}
+
+void run_resolve(const double* sum, const uint32_t* count, int w, int h, int op, double exposure, uint8_t* rgb,
+                 hipStream_t s) {
+  const int n = w * h;
+  hipLaunchKernelGGL(k_resolve, dim3((n + 255) / 256), dim3(256), 0, s, sum, count, n, op, exposure, rgb);
+}
+
+}  // namespace rtg

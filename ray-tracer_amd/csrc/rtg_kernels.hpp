@@ -1,0 +1,583 @@
+// rtg_kernels.hpp — the wavefront path tracer and batch trace kernels,
+// templated on the precision mode. Included by rtg_f32.hip and rtg_f64.hip,
+// each of which instantiates one mode (the FP64 unit is compiled with
+// -ffp-contract=off so it rounds like the reference CPU build).
+//
+// One render pass is processed in chunks of paths; path id
+//   pid = (pixel - p0) * k + (sample - s0)
+// so a chunk covers whole pixels with all k samples of the pass. Per chunk:
+//   generate                       camera rays (renderer.cpp:31-37)
+//   for bounce = 0 .. max_depth-1  (integrator.cpp:101-138)
+//     extend    closest hit of the active queue
+//     shade     emission, scatter, light choice -> shadow queue, Russian
+//               roulette, next ray -> next queue (compacted: one atomic per
+//               wavefront, ballot + mbcnt for the lane offsets)
+//     shadow    any-hit of the shadow queue; unoccluded -> L += contribution
+//   accumulate per-pixel sums in sample order (framebuffer.hpp:31-35)
+// The direct and normals integrators run as one kernel per sample (their
+// recursion is a depth-first walk with an explicit frame stack).
+#pragma once
+#include "rtg_shade.hpp"
+
+namespace rtg {
+
+constexpr int kBlock = 128;
+
+enum StatIdx { kStCamera = 0, kStRays, kStShadow, kStRR, kStInvalid, kStVisits, kStTests, kStCount };
+
+template <class T>
+struct PathBuffers {
+  T *ox, *oy, *oz, *dx, *dy, *dz;  // current ray
+  T *bx, *by, *bz;                 // throughput
+  T *lx, *ly, *lz;                 // radiance
+  uint8_t* spec;                   // specular_bounce
+  T *ht, *hb1, *hb2;               // hit t, barycentrics
+  uint32_t* hprim;                 // hit primitive (0xFFFFFFFF none)
+  int32_t* hplane;                 // hit plane (-1 none)
+  uint64_t *rs, *ri;               // PCG32 state/inc (FP64 mode)
+  uint32_t* q[2];                  // active queues
+  uint32_t* counts;                // [b] queue size at bounce b; [kMaxBounce + b] shadow queue size
+  uint32_t* sq_pid;                // shadow queue
+  T *sox, *soy, *soz, *sdx, *sdy, *sdz, *stm, *scx, *scy, *scz;
+  unsigned long long* stats;       // kStCount counters
+  uint32_t capacity;
+};
+constexpr int kMaxBounce = 1024;
+
+struct ChunkParams {
+  uint32_t width, height;
+  uint32_t p0, k, s0;  // pixels [p0, p0 + n_pix), samples [s0, s0 + k)
+  uint32_t n_pix, n_paths;
+  int32_t max_depth, rr_min_depth, integrator, light_sampling, use_bvh;
+  uint64_t seed;
+  double* acc_sum;      // width*height*3
+  uint32_t* acc_count;  // width*height
+};
+
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0) return 0;
+  const uint32_t lane = __lane_id();
+  const int leader = __ffsll((long long)mask) - 1;
+  uint32_t base = 0;
+  if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// Block-level reduction of per-thread counters, one global atomic per block.
+__device__ __forceinline__ void flush_stats(unsigned long long* g, const uint32_t* v, int n) {
+  __shared__ unsigned long long sh[kStCount];
+  if (threadIdx.x < kStCount) sh[threadIdx.x] = 0;
+  __syncthreads();
+  for (int i = 0; i < n; ++i)
+    if (v[i]) atomicAdd(&sh[i], (unsigned long long)v[i]);
+  __syncthreads();
+  if (threadIdx.x < (unsigned)n && sh[threadIdx.x]) atomicAdd(&g[threadIdx.x], sh[threadIdx.x]);
+}
+
+template <class T>
+struct ModeOps;
+
+template <>
+struct ModeOps<double> {
+  using R = PcgDraws;
+  __device__ static Hit64 closest(const DevScene& S, bool bvh, V3<double> o, V3<double> d, LdsStack, Counters& c) {
+    return closest64(S, bvh, o, d, 1e-6, __builtin_inf(), c);
+  }
+  __device__ static bool occluded(const DevScene& S, bool bvh, V3<double> o, V3<double> d, double tmax, LdsStack,
+                                  Counters& c) {
+    return any64(S, bvh, o, d, 1e-6, tmax, c);
+  }
+  __device__ static void finalize(const DevScene& S, V3<double> o, V3<double> d, double t, double b1, double b2,
+                                  uint32_t prim, int plane, HitRec<double>& rec) {
+    finalize64(S, o, d, t, b1, b2, prim, plane, rec);
+  }
+  __device__ static bool light(const DevScene& S, uint32_t li, const HitRec<double>& rec, const Mat<double>& m, R& rng,
+                               ShadowReq<double>& req) {
+    return sample_light64(S, li, rec, m, rng, req);
+  }
+  __device__ static void camera(const DevScene& S, int x, int y, double sx, double sy, R& rng, V3<double>& o,
+                                V3<double>& d) {
+    camera64(S, x, y, sx, sy, rng, o, d);
+  }
+};
+
+template <>
+struct ModeOps<float> {
+  using R = HashDraws;
+  __device__ static Hit32 closest(const DevScene& S, bool bvh, V3<float> o, V3<float> d, LdsStack stk, Counters& c) {
+    const Ray32 r = make_ray32(o.x, o.y, o.z, d.x, d.y, d.z);
+    return closest32(S, bvh, r, 1e-6f, __builtin_inff(), stk, c);
+  }
+  __device__ static bool occluded(const DevScene& S, bool bvh, V3<float> o, V3<float> d, float tmax, LdsStack stk,
+                                  Counters& c) {
+    const Ray32 r = make_ray32(o.x, o.y, o.z, d.x, d.y, d.z);
+    return any32(S, bvh, r, 1e-6f, tmax, stk, c);
+  }
+  __device__ static void finalize(const DevScene& S, V3<float> o, V3<float> d, float t, float b1, float b2,
+                                  uint32_t prim, int plane, HitRec<float>& rec) {
+    finalize32(S, o, d, t, b1, b2, prim, plane, rec);
+  }
+  __device__ static bool light(const DevScene& S, uint32_t li, const HitRec<float>& rec, const Mat<float>& m, R& rng,
+                               ShadowReq<float>& req) {
+    return sample_light32(S, li, rec, m, rng, req);
+  }
+  __device__ static void camera(const DevScene& S, int x, int y, float sx, float sy, R& rng, V3<float>& o,
+                                V3<float>& d) {
+    camera32(S, x, y, sx, sy, rng, o, d);
+  }
+};
+
+// Draw source of one path at one bounce.
+template <class T>
+struct DrawSrc;
+template <>
+struct DrawSrc<double> {
+  __device__ static PcgDraws make(const PathBuffers<double>& B, const ChunkParams&, uint32_t pid, uint32_t, uint32_t,
+                                  uint32_t) {
+    PcgDraws d;
+    d.pcg.state = B.rs[pid];
+    d.pcg.inc = B.ri[pid];
+    return d;
+  }
+  __device__ static void store(const PathBuffers<double>& B, uint32_t pid, const PcgDraws& d) { B.rs[pid] = d.pcg.state; }
+};
+template <>
+struct DrawSrc<float> {
+  __device__ static HashDraws make(const PathBuffers<float>&, const ChunkParams& P, uint32_t, uint32_t pixel,
+                                   uint32_t sample, uint32_t bounce) {
+    return HashDraws(path_key(P.seed, pixel, sample), bounce);
+  }
+  __device__ static void store(const PathBuffers<float>&, uint32_t, const HashDraws&) {}
+};
+
+template <int STACK>
+__device__ __forceinline__ LdsStack lds_stack(uint32_t* ref, float* tn) {
+  return LdsStack{ref + threadIdx.x, tn + threadIdx.x, (uint32_t)blockDim.x};
+}
+
+// ---------------------------------------------------------------------------
+// generate: one camera ray per path (renderer.cpp:31-37).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_generate(DevScene S, PathBuffers<T> B, ChunkParams P) {
+  const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid == 0) {
+    B.counts[0] = P.n_paths;
+    for (int b = 1; b <= P.max_depth && b < kMaxBounce; ++b) B.counts[b] = 0;
+    for (int b = 0; b < P.max_depth && b < kMaxBounce; ++b) B.counts[kMaxBounce + b] = 0;
+  }
+  if (pid >= P.n_paths) return;
+  const uint32_t pixel = P.p0 + pid / P.k;
+  const uint32_t sample = P.s0 + pid % P.k;
+  const int x = (int)(pixel % P.width), y = (int)(pixel / P.width);
+  V3<T> o, d;
+  if constexpr (sizeof(T) == 8) {
+    PcgDraws rng;
+    rng.pcg = Pcg32::for_sample(P.seed, pixel, sample);
+    const double jx = rng.next();
+    const double jy = rng.next();
+    ModeOps<T>::camera(S, x, y, jx, jy, rng, o, d);
+    B.rs[pid] = rng.pcg.state;
+    B.ri[pid] = rng.pcg.inc;
+  } else {
+    HashDraws rng(path_key(P.seed, pixel, sample), 0xFFFFFFFFu);
+    const float jx = rng.next();
+    const float jy = rng.next();
+    ModeOps<T>::camera(S, x, y, jx, jy, rng, o, d);
+  }
+  B.ox[pid] = o.x; B.oy[pid] = o.y; B.oz[pid] = o.z;
+  B.dx[pid] = d.x; B.dy[pid] = d.y; B.dz[pid] = d.z;
+  B.bx[pid] = 1; B.by[pid] = 1; B.bz[pid] = 1;
+  B.lx[pid] = 0; B.ly[pid] = 0; B.lz[pid] = 0;
+  B.spec[pid] = 1;
+  B.q[0][pid] = pid;
+}
+
+// extend: closest hit for the active queue of bounce b.
+template <class T, int STACK>
+__global__ __launch_bounds__(kBlock) void k_extend(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
+  __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
+  __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = B.counts[b];
+  uint32_t st[kStCount] = {0};
+  if (i < n) {
+    const uint32_t pid = B.q[b & 1][i];
+    const V3<T> o{B.ox[pid], B.oy[pid], B.oz[pid]}, d{B.dx[pid], B.dy[pid], B.dz[pid]};
+    Counters c{0, 0};
+    const auto h = ModeOps<T>::closest(S, P.use_bvh != 0, o, d, lds_stack<STACK>(sref, stn), c);
+    B.ht[pid] = h.t;
+    B.hb1[pid] = h.b1;
+    B.hb2[pid] = h.b2;
+    B.hprim[pid] = h.prim;
+    B.hplane[pid] = h.plane;
+    st[kStRays] = 1;
+    st[kStVisits] = c.visits;
+    st[kStTests] = c.tests;
+  }
+  flush_stats(B.stats, st, kStCount);
+}
+
+// shade: one bounce of path_trace (integrator.cpp:103-137) after the hit.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = B.counts[b];
+  uint32_t st[kStCount] = {0};
+  bool alive = false, shadow = false;
+  uint32_t pid = 0;
+  ShadowReq<T> req;
+  V3<T> nro{0, 0, 0}, nrd{0, 0, 0};
+  if (i < n) {
+    pid = B.q[b & 1][i];
+    const uint32_t pixel = P.p0 + pid / P.k, sample = P.s0 + pid % P.k;
+    const V3<T> o{B.ox[pid], B.oy[pid], B.oz[pid]}, d{B.dx[pid], B.dy[pid], B.dz[pid]};
+    V3<T> beta{B.bx[pid], B.by[pid], B.bz[pid]};
+    V3<T> L{B.lx[pid], B.ly[pid], B.lz[pid]};
+    const bool spec = B.spec[pid] != 0;
+    const uint32_t prim = B.hprim[pid];
+    const int plane = B.hplane[pid];
+    if (prim == 0xFFFFFFFFu && plane < 0) {
+      L = L + beta * background<T>(S, d);
+    } else {
+      auto rng = DrawSrc<T>::make(B, P, pid, pixel, sample, (uint32_t)b);
+      HitRec<T> rec;
+      ModeOps<T>::finalize(S, o, d, B.ht[pid], B.hb1[pid], B.hb2[pid], prim, plane, rec);
+      const Mat<T> mat = load_mat<T>(S, rec.material);
+      const V3<T> Le = emitted(mat, rec.front_face);
+      const bool sampled = !rec.is_plane && S.sampled[rec.prim] != 0;
+      if (!is_zero(Le) && (spec || !P.light_sampling || !sampled)) L = L + beta * Le;
+      Scatter<T> sr;
+      if (scatter(mat, d, rec, rng, sr)) {
+        if (P.light_sampling && !sr.specular && S.n_lights > 0) {
+          const uint32_t nl = S.n_lights;
+          const uint32_t li = rng.below(nl);
+          if (ModeOps<T>::light(S, li, rec, mat, rng, req)) {
+            shadow = true;
+            req.f_rop = (beta * req.f_rop) * T(nl);
+          }
+        }
+        beta = beta * sr.weight;
+        bool live = !is_zero(beta);
+        if (live && b + 1 >= P.rr_min_depth) {
+          T q = max_comp(beta);
+          q = q < T(0.05) ? T(0.05) : (q > T(0.95) ? T(0.95) : q);
+          if (rng.next() >= q) {
+            st[kStRR] = 1;
+            live = false;
+          } else {
+            beta = beta * (T(1) / q);
+          }
+        }
+        if (live) {
+          nro = offset_ray_origin(rec.p, rec.gn, sr.dir);
+          nrd = sr.dir;
+          alive = b + 1 < P.max_depth;
+          B.bx[pid] = beta.x; B.by[pid] = beta.y; B.bz[pid] = beta.z;
+          B.spec[pid] = sr.specular ? 1 : 0;
+        }
+      }
+      DrawSrc<T>::store(B, pid, rng);
+    }
+    B.lx[pid] = L.x; B.ly[pid] = L.y; B.lz[pid] = L.z;
+    if (alive) {
+      B.ox[pid] = nro.x; B.oy[pid] = nro.y; B.oz[pid] = nro.z;
+      B.dx[pid] = nrd.x; B.dy[pid] = nrd.y; B.dz[pid] = nrd.z;
+    }
+  }
+  // Converged: compacted appends.
+  const uint32_t qi = wave_append(&B.counts[b + 1], alive);
+  if (alive) B.q[(b + 1) & 1][qi] = pid;
+  const uint32_t si = wave_append(&B.counts[kMaxBounce + b], shadow);
+  if (shadow) {
+    B.sq_pid[si] = pid;
+    B.sox[si] = req.o.x; B.soy[si] = req.o.y; B.soz[si] = req.o.z;
+    B.sdx[si] = req.d.x; B.sdy[si] = req.d.y; B.sdz[si] = req.d.z;
+    B.stm[si] = req.tmax;
+    B.scx[si] = req.f_rop.x; B.scy[si] = req.f_rop.y; B.scz[si] = req.f_rop.z;
+    st[kStShadow] = 1;
+  }
+  flush_stats(B.stats, st, kStCount);
+}
+
+// shadow: any-hit of the shadow queue of bounce b (Scene::occluded).
+template <class T, int STACK>
+__global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
+  __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
+  __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = B.counts[kMaxBounce + b];
+  uint32_t st[kStCount] = {0};
+  if (i < n) {
+    const uint32_t pid = B.sq_pid[i];
+    const V3<T> o{B.sox[i], B.soy[i], B.soz[i]}, d{B.sdx[i], B.sdy[i], B.sdz[i]};
+    Counters c{0, 0};
+    const bool occ = ModeOps<T>::occluded(S, P.use_bvh != 0, o, d, B.stm[i], lds_stack<STACK>(sref, stn), c);
+    if (!occ) {
+      B.lx[pid] = B.lx[pid] + B.scx[i];
+      B.ly[pid] = B.ly[pid] + B.scy[i];
+      B.lz[pid] = B.lz[pid] + B.scz[i];
+    }
+    st[kStVisits] = c.visits;
+    st[kStTests] = c.tests;
+  }
+  flush_stats(B.stats, st, kStCount);
+}
+
+// accumulate: the invalid-sample guard and Framebuffer::add in sample order
+// (renderer.cpp:39-45, framebuffer.hpp:31-35).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_accumulate(PathBuffers<T> B, ChunkParams P) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t st[kStCount] = {0};
+  if (j < P.n_pix) {
+    const uint32_t pixel = P.p0 + j;
+    double sx = P.acc_sum[3 * (size_t)pixel], sy = P.acc_sum[3 * (size_t)pixel + 1],
+           sz = P.acc_sum[3 * (size_t)pixel + 2];
+    for (uint32_t s = 0; s < P.k; ++s) {
+      const uint32_t pid = j * P.k + s;
+      T x = B.lx[pid], y = B.ly[pid], z = B.lz[pid];
+      if (!(r_isfinite(x) && r_isfinite(y) && r_isfinite(z)) || x < 0 || y < 0 || z < 0) {
+        ++st[kStInvalid];
+        x = y = z = 0;
+      }
+      sx += (double)x;
+      sy += (double)y;
+      sz += (double)z;
+    }
+    P.acc_sum[3 * (size_t)pixel] = sx;
+    P.acc_sum[3 * (size_t)pixel + 1] = sy;
+    P.acc_sum[3 * (size_t)pixel + 2] = sz;
+    P.acc_count[pixel] += P.k;
+    st[kStCamera] = P.k;
+  }
+  flush_stats(B.stats, st, kStCount);
+}
+
+// ---------------------------------------------------------------------------
+// direct / normals integrators: one sample per thread (integrator.cpp:141-198).
+template <class T, int STACK>
+__device__ V3<T> direct_sample(const DevScene& S, const ChunkParams& P, V3<T> o, V3<T> d,
+                               typename ModeOps<T>::R& rng, LdsStack stk, uint32_t* st) {
+  struct Frame {
+    V3<T> L, w1, w2, o2, d2;
+    int phase;  // 0 trace, 1 waiting first child, 2 waiting second child
+    bool second;
+  };
+  constexpr int kMaxFrames = 33;
+  Frame fr[kMaxFrames];
+  int top = 0;
+  fr[0].phase = 0;
+  fr[0].o2 = o;
+  fr[0].d2 = d;
+  V3<T> ret{0, 0, 0};
+  bool have_ret = false;
+  while (top >= 0) {
+    Frame& f = fr[top];
+    if (have_ret) {
+      have_ret = false;
+      if (f.phase == 1) {
+        f.L = f.L + f.w1 * ret;
+        if (f.second) {
+          f.phase = 2;
+          if (top + 1 >= kMaxFrames) { ret = f.L; have_ret = true; --top; continue; }
+          fr[top + 1].phase = 0;
+          fr[top + 1].o2 = f.o2;
+          fr[top + 1].d2 = f.d2;
+          ++top;
+          continue;
+        }
+      } else {
+        f.L = f.L + f.w2 * ret;
+      }
+      ret = f.L;
+      have_ret = true;
+      --top;
+      continue;
+    }
+    // phase 0: trace fr[top].o2/d2 at depth `top`
+    const V3<T> ro = f.o2, rd = f.d2;
+    Counters c{0, 0};
+    ++st[kStRays];
+    const auto h = ModeOps<T>::closest(S, P.use_bvh != 0, ro, rd, stk, c);
+    st[kStVisits] += c.visits;
+    st[kStTests] += c.tests;
+    if (h.prim == 0xFFFFFFFFu && h.plane < 0) {
+      // Normals: a miss is black (integrator.cpp:190); direct: the sky.
+      ret = P.integrator == RTG_INTEGRATOR_NORMALS ? V3<T>{0, 0, 0} : background<T>(S, rd);
+      have_ret = true;
+      --top;
+      continue;
+    }
+    HitRec<T> rec;
+    ModeOps<T>::finalize(S, ro, rd, h.t, h.b1, h.b2, h.prim, h.plane, rec);
+    const Mat<T> mat = load_mat<T>(S, rec.material);
+    f.L = emitted(mat, rec.front_face);
+    if (P.integrator == RTG_INTEGRATOR_NORMALS) {
+      const V3<T> nn = rec.front_face ? rec.normal : -rec.normal;
+      ret = (nn + V3<T>{1, 1, 1}) * T(0.5);
+      have_ret = true;
+      --top;
+      continue;
+    }
+    if (top >= P.max_depth) {
+      ret = f.L;
+      have_ret = true;
+      --top;
+      continue;
+    }
+    if (mat.kind == RTG_MAT_LAMBERTIAN) {
+      for (uint32_t li = 0; li < S.n_lights; ++li) {
+        ShadowReq<T> req;
+        if (ModeOps<T>::light(S, li, rec, mat, rng, req)) {
+          ++st[kStShadow];
+          Counters c2{0, 0};
+          if (!ModeOps<T>::occluded(S, P.use_bvh != 0, req.o, req.d, req.tmax, stk, c2)) f.L = f.L + req.f_rop;
+          st[kStVisits] += c2.visits;
+          st[kStTests] += c2.tests;
+        }
+      }
+      ret = f.L;
+      have_ret = true;
+      --top;
+      continue;
+    }
+    if (mat.kind == RTG_MAT_METAL) {
+      Scatter<T> sr;
+      if (scatter(mat, rd, rec, rng, sr) && top + 1 < kMaxFrames) {
+        f.w1 = mat.c0;
+        f.second = false;
+        f.phase = 1;
+        fr[top + 1].phase = 0;
+        fr[top + 1].o2 = offset_ray_origin(rec.p, rec.gn, sr.dir);
+        fr[top + 1].d2 = sr.dir;
+        ++top;
+        continue;
+      }
+      ret = f.L;
+      have_ret = true;
+      --top;
+      continue;
+    }
+    if (mat.kind == RTG_MAT_DIELECTRIC && top + 1 < kMaxFrames) {
+      const V3<T> in = normalize(rd);
+      const T eta = rec.front_face ? T(1) / mat.p0 : mat.p0;
+      const T cos_i = r_fmin(dot(-in, rec.normal), T(1));
+      V3<T> refracted;
+      const bool can = refract(in, rec.normal, eta, &refracted);
+      const T F = can ? schlick(cos_i, eta) : T(1);
+      const V3<T> rdir = reflect(in, rec.normal);
+      f.w1 = mat.c0 * F;
+      f.second = can;
+      if (can) {
+        const V3<T> tdir = normalize(refracted);
+        f.w2 = mat.c0 * (T(1) - F);
+        f.o2 = offset_ray_origin(rec.p, rec.gn, tdir);
+        f.d2 = tdir;
+      }
+      f.phase = 1;
+      fr[top + 1].phase = 0;
+      fr[top + 1].o2 = offset_ray_origin(rec.p, rec.gn, rdir);
+      fr[top + 1].d2 = rdir;
+      ++top;
+      continue;
+    }
+    ret = f.L;
+    have_ret = true;
+    --top;
+  }
+  return ret;
+}
+
+template <class T, int STACK>
+__global__ __launch_bounds__(kBlock) void k_direct(DevScene S, PathBuffers<T> B, ChunkParams P) {
+  __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
+  __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
+  const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t st[kStCount] = {0};
+  if (pid < P.n_paths) {
+    const uint32_t pixel = P.p0 + pid / P.k, sample = P.s0 + pid % P.k;
+    const int x = (int)(pixel % P.width), y = (int)(pixel / P.width);
+    V3<T> o, d, L;
+    if constexpr (sizeof(T) == 8) {
+      PcgDraws rng;
+      rng.pcg = Pcg32::for_sample(P.seed, pixel, sample);
+      const double jx = rng.next();
+      const double jy = rng.next();
+      ModeOps<T>::camera(S, x, y, jx, jy, rng, o, d);
+      L = direct_sample<T, STACK>(S, P, o, d, rng, lds_stack<STACK>(sref, stn), st);
+    } else {
+      HashDraws rng(path_key(P.seed, pixel, sample), 0xFFFFFFFFu);
+      const float jx = rng.next();
+      const float jy = rng.next();
+      ModeOps<T>::camera(S, x, y, jx, jy, rng, o, d);
+      rng.bounce = 0;
+      rng.idx = 0;
+      L = direct_sample<T, STACK>(S, P, o, d, rng, lds_stack<STACK>(sref, stn), st);
+    }
+    B.lx[pid] = L.x;
+    B.ly[pid] = L.y;
+    B.lz[pid] = L.z;
+  }
+  flush_stats(B.stats, st, kStCount);
+}
+
+// ---------------------------------------------------------------------------
+// Batch ray queries (rtg_trace): rays {o, d, tmin, tmax} as doubles.
+template <class T, int STACK>
+__global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const double* __restrict__ rays, uint64_t n, int mode,
+                                                  int use_bvh, rtg_hit* __restrict__ hits,
+                                                  unsigned long long* stats) {
+  __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
+  __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t st[kStCount] = {0};
+  if (i < n) {
+    const double* r = rays + 8 * i;
+    rtg_hit out;
+    out.t = __builtin_inf();
+    out.prim = 0xFFFFFFFFu;
+    out.flags = 0;
+    Counters c{0, 0};
+    const LdsStack stk = lds_stack<STACK>(sref, stn);
+    if constexpr (sizeof(T) == 8) {
+      const V3<double> o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
+      if (mode == RTG_TRACE_CLOSEST) {
+        const Hit64 h = closest64(S, use_bvh != 0, o, d, r[6], r[7], c);
+        if (h.prim != 0xFFFFFFFFu || h.plane >= 0) {
+          HitRec<double> rec;
+          finalize64(S, o, d, h.t, h.b1, h.b2, h.prim, h.plane, rec);
+          out.t = h.t;
+          out.prim = h.prim != 0xFFFFFFFFu ? h.prim : (uint32_t)h.plane;
+          out.flags = 1u | (h.prim == 0xFFFFFFFFu ? 2u : 0u) | (rec.front_face ? 4u : 0u);
+        }
+      } else {
+        out.flags = any64(S, use_bvh != 0, o, d, 1e-6, r[7], c) ? 1u : 0u;
+      }
+    } else {
+      const Ray32 ray = make_ray32((float)r[0], (float)r[1], (float)r[2], (float)r[3], (float)r[4], (float)r[5]);
+      if (mode == RTG_TRACE_CLOSEST) {
+        const Hit32 h = closest32(S, use_bvh != 0, ray, (float)r[6], (float)r[7], stk, c);
+        if (h.prim != 0xFFFFFFFFu || h.plane >= 0) {
+          HitRec<float> rec;
+          finalize32(S, V3<float>{ray.ox, ray.oy, ray.oz}, V3<float>{ray.dx, ray.dy, ray.dz}, h.t, h.b1, h.b2,
+                     h.prim, h.plane, rec);
+          out.t = h.t;
+          out.prim = h.prim != 0xFFFFFFFFu ? h.prim : (uint32_t)h.plane;
+          out.flags = 1u | (h.prim == 0xFFFFFFFFu ? 2u : 0u) | (rec.front_face ? 4u : 0u);
+        }
+      } else {
+        out.flags = any32(S, use_bvh != 0, ray, 1e-6f, (float)r[7], stk, c) ? 1u : 0u;
+      }
+    }
+    hits[i] = out;
+    st[kStRays] = 1;
+    st[kStVisits] = c.visits;
+    st[kStTests] = c.tests;
+  }
+  flush_stats(stats, st, kStCount);
+}
+
+}  // namespace rtg

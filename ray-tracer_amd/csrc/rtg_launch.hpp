@@ -1,0 +1,27 @@
+// rtg_launch.hpp — host-side launchers implemented by the two kernel
+// translation units (rtg_f32.hip, rtg_f64.hip) and called by rtg_api.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "rtg_kernels.hpp"
+
+namespace rtg {
+
+// Stack classes of the FP32 traversal kernels (entries per lane in LDS).
+constexpr uint32_t kStackClasses[3] = {20, 32, 64};
+
+// Runs one chunk of a render pass: generate, max_depth x (extend, shade,
+// shadow), accumulate; or the per-sample direct/normals kernel.
+void run_chunk(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P, uint32_t stack, hipStream_t s);
+void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t stack, hipStream_t s);
+
+void run_trace32(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,
+                 unsigned long long* stats, uint32_t stack, hipStream_t s);
+void run_trace64(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,
+                 unsigned long long* stats, hipStream_t s);
+
+// Tone map + sRGB8 (tonemap.cpp:14-53), FP64 like the reference.
+void run_resolve(const double* sum, const uint32_t* count, int w, int h, int op, double exposure, uint8_t* rgb,
+                 hipStream_t s);
+
+}  // namespace rtg

@@ -1,0 +1,500 @@
+// rtg_trace.hpp — ray/primitive tests and BVH traversal on the device.
+//
+// Semantics follow Scene::intersect / occluded / intersect_brute
+// (src/core/scene.cpp:139-234) and Bvh::closest / any
+// (include/raytracer/geometry/bvh.hpp:75-132):
+//   * planes are tested first, outside the BVH, and bound t_max;
+//   * closest-hit visits the near child first (by the sign of the ray
+//     direction on the node's split axis) and keeps the first strictly
+//     closer hit; any-hit returns on the first hit;
+//   * node_visits counts node boxes tested, prim_tests primitive tests.
+// FP64 functions are the reference algorithm on the reference layout and
+// reproduce its hits bit for bit. FP32 functions run on the two-child
+// layout: both child boxes are tested from one 64-byte record, the far
+// child is pushed with its entry distance and skipped on pop if a closer
+// hit has arrived since (the reference's pop-time box test, without
+// reloading the box).
+#pragma once
+#include "rtg_math.hpp"
+#include "rtg_scene.hpp"
+
+namespace rtg {
+
+struct Counters {
+  uint32_t visits, tests;
+};
+
+// =========================== FP64 (exact) ===============================
+
+__device__ __forceinline__ V3<double> d3(rtg_vec3 v) { return V3<double>{v.x, v.y, v.z}; }
+
+// aabb.hpp:57-66
+__device__ __forceinline__ bool slab64(double lo, double hi, double o, double inv, double& tmin, double& tmax) {
+  double t0 = (lo - o) * inv;
+  double t1 = (hi - o) * inv;
+  if (t0 != t0 || t1 != t1) return true;
+  if (t0 > t1) {
+    const double t = t0;
+    t0 = t1;
+    t1 = t;
+  }
+  tmin = t0 > tmin ? t0 : tmin;
+  tmax = t1 < tmax ? t1 : tmax;
+  return tmin <= tmax;
+}
+__device__ __forceinline__ bool box64(const rtg_bvh_node& n, V3<double> o, V3<double> inv, double tmin, double tmax) {
+  return slab64(n.lo.x, n.hi.x, o.x, inv.x, tmin, tmax) && slab64(n.lo.y, n.hi.y, o.y, inv.y, tmin, tmax) &&
+         slab64(n.lo.z, n.hi.z, o.z, inv.z, tmin, tmax);
+}
+
+// shapes.hpp:25-46
+__device__ __forceinline__ bool sphere_hit64(V3<double> c, double radius, V3<double> o, V3<double> d, double tmin,
+                                             double tmax, double& t_out) {
+  const V3<double> oc = c - o;
+  const double a = len2(d);
+  if (a == 0) return false;
+  const double h = dot(d, oc);
+  const double cc = len2(oc) - radius * radius;
+  const V3<double> v = oc - d * (h / a);
+  const double disc = a * (radius * radius - len2(v));
+  if (disc < 0) return false;
+  const double sq = r_sqrt(disc);
+  const double q = h + r_copysign(sq, h);
+  double t0 = cc / q, t1 = q / a;
+  if (q == 0) { t0 = t1 = h / a; }
+  if (t0 > t1) {
+    const double t = t0;
+    t0 = t1;
+    t1 = t;
+  }
+  double t = t0;
+  if (!(tmin < t && t < tmax)) {
+    t = t1;
+    if (!(tmin < t && t < tmax)) return false;
+  }
+  t_out = t;
+  return true;
+}
+
+// shapes.hpp:110-127
+__device__ __forceinline__ bool tri_hit64(const rtg_primitive& p, V3<double> o, V3<double> d, double tmin,
+                                          double tmax, double& t_out, double& b1o, double& b2o) {
+  const V3<double> v0 = d3(p.v0);
+  const V3<double> e1 = d3(p.v1) - v0, e2 = d3(p.v2) - v0;
+  const V3<double> pvec = cross(d, e2);
+  const double det = dot(e1, pvec);
+  if (r_fabs(det) < 1e-14) return false;
+  const double inv_det = 1.0 / det;
+  const V3<double> tvec = o - v0;
+  const double b1 = dot(tvec, pvec) * inv_det;
+  if (b1 < 0 || b1 > 1) return false;
+  const V3<double> qvec = cross(tvec, e1);
+  const double b2 = dot(d, qvec) * inv_det;
+  if (b2 < 0 || b1 + b2 > 1) return false;
+  const double t = dot(e2, qvec) * inv_det;
+  if (!(tmin < t && t < tmax)) return false;
+  t_out = t;
+  b1o = b1;
+  b2o = b2;
+  return true;
+}
+
+__device__ __forceinline__ bool prim_hit64(const rtg_primitive& p, V3<double> o, V3<double> d, double tmin,
+                                           double tmax, double& t, double& b1, double& b2) {
+  if (p.kind == RTG_PRIM_SPHERE) return sphere_hit64(d3(p.v0), p.v1.x, o, d, tmin, tmax, t);
+  return tri_hit64(p, o, d, tmin, tmax, t, b1, b2);
+}
+
+// shapes.hpp:82-86 (t only); returns the plane index hit last (planes loop
+// keeps the last closer one) or -1.
+__device__ __forceinline__ int planes64(const DevScene& S, V3<double> o, V3<double> d, double tmin, double& tmax) {
+  int hit = -1;
+  for (uint32_t i = 0; i < S.n_planes; ++i) {
+    const rtg_plane& pl = S.planes[i];
+    const V3<double> n = d3(pl.normal);
+    const double denom = dot(n, d);
+    if (r_fabs(denom) < 1e-12) continue;
+    const double t = dot(d3(pl.point) - o, n) / denom;
+    if (!(tmin < t && t < tmax)) continue;
+    tmax = t;
+    hit = (int)i;
+  }
+  return hit;
+}
+
+struct Hit64 {
+  double t, b1, b2;
+  uint32_t prim;  // 0xFFFFFFFF none
+  int plane;      // -1 none
+};
+
+// Scene::intersect (scene.cpp:176-198) without finalize.
+__device__ inline Hit64 closest64(const DevScene& S, bool use_bvh, V3<double> o, V3<double> d, double tmin,
+                                  double tmax, Counters& c) {
+  Hit64 h;
+  h.plane = planes64(S, o, d, tmin, tmax);
+  h.t = tmax;
+  h.prim = 0xFFFFFFFFu;
+  h.b1 = h.b2 = 0;
+  if (!use_bvh) {  // scene.cpp:200-215
+    for (uint32_t i = 0; i < S.n_prims; ++i) {
+      ++c.tests;
+      double th, b1, b2;
+      if (prim_hit64(S.prims[i], o, d, tmin, h.t, th, b1, b2)) {
+        h.t = th;
+        h.b1 = b1;
+        h.b2 = b2;
+        h.prim = i;
+      }
+    }
+    if (h.prim == 0xFFFFFFFFu && h.plane >= 0) h.t = tmax;
+    return h;
+  }
+  if (S.n_nodes == 0) return h;
+  const V3<double> inv{1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
+  const bool neg[3] = {inv.x < 0, inv.y < 0, inv.z < 0};
+  uint32_t stack[64];
+  int sp = 0;
+  uint32_t idx = 0;
+  double tm = tmax;
+  for (;;) {
+    const rtg_bvh_node n = S.nodes[idx];
+    ++c.visits;
+    if (box64(n, o, inv, tmin, tm)) {
+      if (n.count > 0) {
+        for (uint32_t i = 0; i < n.count; ++i) {
+          ++c.tests;
+          const uint32_t pi = S.prim_indices[n.offset + i];
+          double th, b1, b2;
+          if (prim_hit64(S.prims[pi], o, d, tmin, tm, th, b1, b2)) {
+            h.t = th;
+            h.b1 = b1;
+            h.b2 = b2;
+            h.prim = pi;
+            tm = th;
+          }
+        }
+      } else {
+        if (neg[n.axis]) {
+          stack[sp++] = idx + 1;
+          idx = n.offset;
+        } else {
+          stack[sp++] = n.offset;
+          idx = idx + 1;
+        }
+        continue;
+      }
+    }
+    if (sp == 0) break;
+    idx = stack[--sp];
+  }
+  return h;
+}
+
+// Scene::occluded (scene.cpp:217-234); t_min is the caller's (1e-6 there).
+__device__ inline bool any64(const DevScene& S, bool use_bvh, V3<double> o, V3<double> d, double tmin, double tmax,
+                             Counters& c) {
+  for (uint32_t i = 0; i < S.n_planes; ++i) {
+    const rtg_plane& pl = S.planes[i];
+    const V3<double> n = d3(pl.normal);
+    const double denom = dot(n, d);
+    if (r_fabs(denom) < 1e-12) continue;
+    const double t = dot(d3(pl.point) - o, n) / denom;
+    if (tmin < t && t < tmax) return true;
+  }
+  double th, b1, b2;
+  if (!use_bvh) {
+    for (uint32_t i = 0; i < S.n_prims; ++i) {
+      ++c.tests;
+      if (prim_hit64(S.prims[i], o, d, tmin, tmax, th, b1, b2)) return true;
+    }
+    return false;
+  }
+  if (S.n_nodes == 0) return false;
+  const V3<double> inv{1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
+  uint32_t stack[64];
+  int sp = 0;
+  uint32_t idx = 0;
+  for (;;) {
+    const rtg_bvh_node n = S.nodes[idx];
+    ++c.visits;
+    if (box64(n, o, inv, tmin, tmax)) {
+      if (n.count > 0) {
+        for (uint32_t i = 0; i < n.count; ++i) {
+          ++c.tests;
+          if (prim_hit64(S.prims[S.prim_indices[n.offset + i]], o, d, tmin, tmax, th, b1, b2)) return true;
+        }
+      } else {
+        stack[sp++] = n.offset;
+        idx = idx + 1;
+        continue;
+      }
+    }
+    if (sp == 0) break;
+    idx = stack[--sp];
+  }
+  return false;
+}
+
+// =========================== FP32 (fast) ================================
+
+struct Ray32 {
+  float ox, oy, oz, dx, dy, dz, ix, iy, iz;
+};
+
+__device__ __forceinline__ float safe_inv(float d) {
+  // |d| < 1e-30 is replaced by +-1e-30 so slab distances stay finite (no
+  // 0 * inf); boxes still bound the ray conservatively.
+  const float a = r_fabs(d) < 1e-30f ? r_copysign(1e-30f, d) : d;
+  return 1.0f / a;
+}
+
+__device__ __forceinline__ Ray32 make_ray32(float ox, float oy, float oz, float dx, float dy, float dz) {
+  Ray32 r;
+  r.ox = ox; r.oy = oy; r.oz = oz;
+  r.dx = dx; r.dy = dy; r.dz = dz;
+  r.ix = safe_inv(dx); r.iy = safe_inv(dy); r.iz = safe_inv(dz);
+  return r;
+}
+
+// Slab test of one float box; returns the entry distance in tn.
+__device__ __forceinline__ bool box32(const Ray32& r, float lx, float ly, float lz, float hx, float hy, float hz,
+                                      float tmin, float tmax, float& tn) {
+  const float t0x = (lx - r.ox) * r.ix, t1x = (hx - r.ox) * r.ix;
+  const float t0y = (ly - r.oy) * r.iy, t1y = (hy - r.oy) * r.iy;
+  const float t0z = (lz - r.oz) * r.iz, t1z = (hz - r.oz) * r.iz;
+  const float nx = r_fmin(t0x, t1x), fx = r_fmax(t0x, t1x);
+  const float ny = r_fmin(t0y, t1y), fy = r_fmax(t0y, t1y);
+  const float nz = r_fmin(t0z, t1z), fz = r_fmax(t0z, t1z);
+  tn = r_fmax(r_fmax(nx, ny), r_fmax(nz, tmin));
+  const float tf = r_fmin(r_fmin(fx, fy), r_fmin(fz, tmax));
+  return tn <= tf;
+}
+
+// Slot test (triangle: Möller–Trumbore on precomputed edges; sphere: the
+// stable quadratic). Writes t, barycentrics when closer than tmax.
+__device__ __forceinline__ bool slot_hit32(const float4* __restrict__ slot32, uint32_t k, const Ray32& r, float tmin,
+                                           float tmax, float& t_out, float& b1o, float& b2o, uint32_t& prim) {
+  const float4 a = slot32[k * kSlot32Stride + 0];
+  const float4 b = slot32[k * kSlot32Stride + 1];
+  const uint32_t id = __float_as_uint(a.w);
+  prim = id & 0x7FFFFFFFu;
+  if (!(id & kSlotSphere)) {
+    const float4 c = slot32[k * kSlot32Stride + 2];
+    const V3<float> d{r.dx, r.dy, r.dz};
+    const V3<float> e1{b.x, b.y, b.z}, e2{c.x, c.y, c.z};
+    const V3<float> pvec = cross(d, e2);
+    const float det = dot(e1, pvec);
+    const float inv_det = r_rcp(det);
+    const V3<float> tvec{r.ox - a.x, r.oy - a.y, r.oz - a.z};
+    const float b1 = dot(tvec, pvec) * inv_det;
+    const V3<float> qvec = cross(tvec, e1);
+    const float b2 = dot(d, qvec) * inv_det;
+    const float t = dot(e2, qvec) * inv_det;
+    const bool ok = r_fabs(det) >= 1e-14f && b1 >= 0.0f && b1 <= 1.0f && b2 >= 0.0f && b1 + b2 <= 1.0f &&
+                    tmin < t && t < tmax;
+    if (ok) {
+      t_out = t;
+      b1o = b1;
+      b2o = b2;
+    }
+    return ok;
+  }
+  // sphere: centre a.xyz, radius b.x, r^2 b.y
+  const V3<float> d{r.dx, r.dy, r.dz};
+  const V3<float> oc{a.x - r.ox, a.y - r.oy, a.z - r.oz};
+  const float aa = len2(d);
+  const float h = dot(d, oc);
+  const float cc = len2(oc) - b.y;
+  const V3<float> v = oc - d * (h / aa);
+  const float disc = aa * (b.y - len2(v));
+  if (!(disc >= 0.0f) || aa == 0.0f) return false;
+  const float q = h + r_copysign(r_sqrt(disc), h);
+  float t0 = cc / q, t1 = q / aa;
+  if (q == 0.0f) t0 = t1 = h / aa;
+  if (t0 > t1) {
+    const float t = t0;
+    t0 = t1;
+    t1 = t;
+  }
+  float t = t0;
+  if (!(tmin < t && t < tmax)) {
+    t = t1;
+    if (!(tmin < t && t < tmax)) return false;
+  }
+  t_out = t;
+  b1o = b2o = 0.0f;
+  return true;
+}
+
+__device__ __forceinline__ int planes32(const DevScene& S, const Ray32& r, float tmin, float& tmax) {
+  int hit = -1;
+  for (uint32_t i = 0; i < S.n_planes; ++i) {
+    const float4 p = S.plane32[2 * i], n = S.plane32[2 * i + 1];
+    const float denom = n.x * r.dx + n.y * r.dy + n.z * r.dz;
+    if (r_fabs(denom) < 1e-12f) continue;
+    const float t = ((p.x - r.ox) * n.x + (p.y - r.oy) * n.y + (p.z - r.oz) * n.z) / denom;
+    if (!(tmin < t && t < tmax)) continue;
+    tmax = t;
+    hit = (int)i;
+  }
+  return hit;
+}
+
+struct Hit32 {
+  float t, b1, b2;
+  uint32_t prim;  // 0xFFFFFFFF none
+  int plane;
+};
+
+// LDS stack: entry k of this lane lives at [k * stride].
+struct LdsStack {
+  uint32_t* ref;
+  float* tn;
+  uint32_t stride;
+};
+
+__device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& r, float tmin, float tmax,
+                                  LdsStack stk, Counters& c) {
+  Hit32 h;
+  h.plane = planes32(S, r, tmin, tmax);
+  h.t = tmax;
+  h.prim = 0xFFFFFFFFu;
+  h.b1 = h.b2 = 0.0f;
+  if (!use_bvh) {
+    for (uint32_t i = 0; i < S.n_prims; ++i) {
+      ++c.tests;
+      float t, b1, b2;
+      uint32_t prim;
+      if (slot_hit32(S.slot32, S.prim_slot[i], r, tmin, h.t, t, b1, b2, prim)) {
+        h.t = t;
+        h.b1 = b1;
+        h.b2 = b2;
+        h.prim = prim;
+      }
+    }
+    return h;
+  }
+  if (!S.has_bvh) return h;
+  float tn;
+  ++c.visits;
+  if (!box32(r, S.root_lo[0], S.root_lo[1], S.root_lo[2], S.root_hi[0], S.root_hi[1], S.root_hi[2], tmin, h.t, tn))
+    return h;
+  const bool neg[3] = {r.dx < 0.0f, r.dy < 0.0f, r.dz < 0.0f};
+  uint32_t ref = S.root_ref;
+  uint32_t sp = 0;
+  for (;;) {
+    if (!(ref & kLeafBit)) {
+      const float4* n = S.node2 + (size_t)ref * kNode2Stride;
+      const float4 a = n[0], b = n[1], cc = n[2];
+      const uint4 m = *reinterpret_cast<const uint4*>(n + 3);
+      float tl, tr;
+      const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, h.t, tl);
+      const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, h.t, tr);
+      c.visits += 2;
+      const bool right_first = neg[m.z];
+      const uint32_t first = right_first ? m.y : m.x, second = right_first ? m.x : m.y;
+      const bool hf = right_first ? hr : hl, hs = right_first ? hl : hr;
+      const float ts = right_first ? tl : tr;
+      if (hf) {
+        if (hs) {
+          stk.ref[sp * stk.stride] = second;
+          stk.tn[sp * stk.stride] = ts;
+          ++sp;
+        }
+        ref = first;
+        continue;
+      }
+      if (hs) {
+        ref = second;
+        continue;
+      }
+    } else {
+      const uint32_t k0 = leaf_first(ref), kn = k0 + leaf_count(ref);
+      for (uint32_t k = k0; k < kn; ++k) {
+        ++c.tests;
+        float t, b1, b2;
+        uint32_t prim;
+        if (slot_hit32(S.slot32, k, r, tmin, h.t, t, b1, b2, prim)) {
+          h.t = t;
+          h.b1 = b1;
+          h.b2 = b2;
+          h.prim = prim;
+        }
+      }
+    }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      if (stk.tn[sp * stk.stride] <= h.t) {
+        ref = stk.ref[sp * stk.stride];
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+  }
+  return h;
+}
+
+__device__ inline bool any32(const DevScene& S, bool use_bvh, const Ray32& r, float tmin, float tmax, LdsStack stk,
+                             Counters& c) {
+  for (uint32_t i = 0; i < S.n_planes; ++i) {
+    const float4 p = S.plane32[2 * i], n = S.plane32[2 * i + 1];
+    const float denom = n.x * r.dx + n.y * r.dy + n.z * r.dz;
+    if (r_fabs(denom) < 1e-12f) continue;
+    const float t = ((p.x - r.ox) * n.x + (p.y - r.oy) * n.y + (p.z - r.oz) * n.z) / denom;
+    if (tmin < t && t < tmax) return true;
+  }
+  float t, b1, b2;
+  uint32_t prim;
+  if (!use_bvh) {
+    for (uint32_t i = 0; i < S.n_prims; ++i) {
+      ++c.tests;
+      if (slot_hit32(S.slot32, S.prim_slot[i], r, tmin, tmax, t, b1, b2, prim)) return true;
+    }
+    return false;
+  }
+  if (!S.has_bvh) return false;
+  float tn;
+  ++c.visits;
+  if (!box32(r, S.root_lo[0], S.root_lo[1], S.root_lo[2], S.root_hi[0], S.root_hi[1], S.root_hi[2], tmin, tmax, tn))
+    return false;
+  uint32_t ref = S.root_ref;
+  uint32_t sp = 0;
+  for (;;) {
+    if (!(ref & kLeafBit)) {
+      const float4* n = S.node2 + (size_t)ref * kNode2Stride;
+      const float4 a = n[0], b = n[1], cc = n[2];
+      const uint4 m = *reinterpret_cast<const uint4*>(n + 3);
+      float tl, tr;
+      const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, tmax, tl);
+      const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, tmax, tr);
+      c.visits += 2;
+      if (hl) {  // Bvh::any visits idx+1 (left) first, pushes the right child
+        if (hr) {
+          stk.ref[sp * stk.stride] = m.y;
+          ++sp;
+        }
+        ref = m.x;
+        continue;
+      }
+      if (hr) {
+        ref = m.y;
+        continue;
+      }
+    } else {
+      const uint32_t k0 = leaf_first(ref), kn = k0 + leaf_count(ref);
+      for (uint32_t k = k0; k < kn; ++k) {
+        ++c.tests;
+        if (slot_hit32(S.slot32, k, r, tmin, tmax, t, b1, b2, prim)) return true;
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    ref = stk.ref[sp * stk.stride];
+  }
+  return false;
+}
+
+}  // namespace rtg

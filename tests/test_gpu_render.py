@@ -1,0 +1,134 @@
+"""GPU parity of the wavefront path tracer (rtg_render_pass).
+
+* FP64 mode consumes the reference's PCG32 stream in double: its per-pixel
+  sums must equal the oracle's to ~1e-9 relative (device sin/cos may differ
+  from glibc in the last bit, which moves a path's direction by ~1e-16).
+* FP32 mode (counter-hash RNG) must converge to the closed forms of
+  tests/test_render.cpp:89-136 and agree with the oracle's image statistics.
+"""
+import numpy as np
+import pytest
+
+import orc
+import scenes
+from _pkg import rtg
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_render(ctx, o, s0, s1, precision, passes=None):
+    cfg = o.settings(precision)
+    dev = ctx.scene(o.desc)
+    acc = ctx.accum(cfg.width, cfg.height)
+    stats = []
+    for a, b in (passes or [(s0, s1)]):
+        st, cancelled = ctx.render_pass(dev, cfg, a, b, acc)
+        assert not cancelled
+        stats.append(st.as_dict())
+    s, c = acc.read()
+    acc.release()
+    dev.release()
+    return s, c, stats
+
+
+def rel_close_fraction(a, b, rtol):
+    den = np.maximum(np.abs(b), 1e-12)
+    return (np.abs(a - b) <= rtol * den + 1e-12).mean()
+
+
+@pytest.mark.parametrize("name,w,h,spp", [("cornell_box.json", 24, 24, 4), ("first_light.json", 24, 16, 3),
+                                          ("materials.json", 24, 12, 3), ("sphere_field.json", 24, 16, 2),
+                                          ("soft_shadows.json", 24, 16, 2), ("cornell_glass_metal.json", 24, 24, 2)])
+def test_f64_render_matches_oracle(ctx, name, w, h, spp):
+    j = scenes.load_asset(name)
+    o = orc.OracleScene(j, width=w, height=h)
+    so, co, sto = o.render(0, spp)
+    sg, cg, stg = gpu_render(ctx, o, 0, spp, rtg.PRECISION_F64)
+    assert np.array_equal(cg, co)
+    frac = rel_close_fraction(sg, so, 1e-9)
+    print(name, "fraction within 1e-9:", frac, "max abs diff", np.abs(sg - so).max(), flush=True)
+    assert frac >= 0.99
+    # Ray counts follow the reference's counting rules.
+    assert abs(stg[0]["rays"] - sto["rays"]) <= max(2, 0.002 * sto["rays"])
+    assert abs(stg[0]["shadow_rays"] - sto["shadow_rays"]) <= max(2, 0.002 * sto["shadow_rays"] + 2)
+
+
+def test_f64_render_passes_accumulate_in_sample_order(ctx):
+    o = orc.OracleScene(scenes.load_asset("cornell_box.json"), width=16, height=16)
+    s1, c1, _ = gpu_render(ctx, o, 0, 8, rtg.PRECISION_F64)
+    s2, c2, _ = gpu_render(ctx, o, 0, 8, rtg.PRECISION_F64, passes=[(0, 1), (1, 2), (2, 4), (4, 8)])
+    assert np.array_equal(c1, c2)
+    assert np.array_equal(s1, s2)  # bitwise: same samples, same order
+
+
+def test_f32_render_deterministic(ctx):
+    o = orc.OracleScene(scenes.load_asset("cornell_box.json"), width=32, height=32)
+    a = gpu_render(ctx, o, 0, 8, rtg.PRECISION_F32)
+    b = gpu_render(ctx, o, 0, 8, rtg.PRECISION_F32, passes=[(0, 1), (1, 2), (2, 4), (4, 8)])
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    o2 = orc.OracleScene(scenes.load_asset("cornell_box.json"), width=32, height=32, seed=99)
+    c = gpu_render(ctx, o2, 0, 8, rtg.PRECISION_F32)
+    assert not np.array_equal(a[0], c[0])
+
+
+def mean_lum(s, c):
+    return float((s[..., 1] / c).mean())
+
+
+@pytest.mark.parametrize("precision", [rtg.PRECISION_F32, rtg.PRECISION_F64])
+@pytest.mark.parametrize("nee", [True, False])
+def test_sphere_light_over_plane_converges(ctx, precision, nee):
+    o = orc.OracleScene(scenes.sphere_light_over_plane(nee))
+    spp = o.R["spp"]
+    s, c, _ = gpu_render(ctx, o, 0, spp, precision)
+    assert mean_lum(s, c) == pytest.approx(0.2, rel=0.01 if nee else 0.03)
+
+
+@pytest.mark.parametrize("nee", [True, False])
+def test_ball_in_two_sided_shell(ctx, nee):
+    o = orc.OracleScene(scenes.ball_in_shell(nee))
+    s, c, _ = gpu_render(ctx, o, 0, o.R["spp"], rtg.PRECISION_F32)
+    assert mean_lum(s, c) == pytest.approx(1.2, rel=0.01)
+
+
+def test_uniform_sky(ctx):
+    o = orc.OracleScene(scenes.sky_over_plane())
+    s, c, _ = gpu_render(ctx, o, 0, o.R["spp"], rtg.PRECISION_F32)
+    assert mean_lum(s, c) == pytest.approx(0.4, rel=0.02)
+
+
+def test_f32_cornell_mean_within_3_sigma_of_oracle(ctx):
+    """Per-channel image mean of the FP32 GPU render vs the oracle's
+    (reference-exact) render of the same scene, 3 sigma of the difference."""
+    o = orc.OracleScene(scenes.load("cornell_c1.json"), width=32, height=32)
+    spp = 256
+    so, co, _ = o.render(0, spp)
+    sg, cg, _ = gpu_render(ctx, o, 0, spp, rtg.PRECISION_F32)
+    mo, mg = so / co[..., None], sg / cg[..., None]
+    # sigma of the image mean from per-pixel variation across two halves
+    s_a, c_a, _ = gpu_render(ctx, o, 0, spp // 2, rtg.PRECISION_F32)
+    s_b, c_b, _ = gpu_render(ctx, o, spp // 2, spp, rtg.PRECISION_F32)
+    half_diff = (s_a / c_a[..., None] - s_b / c_b[..., None]).reshape(-1, 3)
+    sigma_pix = half_diff.std(axis=0) / 2 * np.sqrt(2)  # per-pixel sd of a spp-sample mean
+    sigma_mean = sigma_pix / np.sqrt(mo.shape[0] * mo.shape[1]) * np.sqrt(2)
+    d = mg.reshape(-1, 3).mean(0) - mo.reshape(-1, 3).mean(0)
+    print("mean diff", d, "sigma", sigma_mean, "rmse", np.sqrt(((mg - mo) ** 2).mean()), flush=True)
+    assert (np.abs(d) <= 3 * sigma_mean + 1e-9).all()
+
+
+@pytest.mark.parametrize("integrator", ["direct", "normals"])
+def test_direct_and_normals_f64_match_oracle(ctx, integrator):
+    j = scenes.load_asset("materials.json")
+    o = orc.OracleScene(j, width=20, height=12, integrator=integrator)
+    so, co, _ = o.render(0, 2)
+    sg, cg, _ = gpu_render(ctx, o, 0, 2, rtg.PRECISION_F64)
+    assert np.array_equal(cg, co)
+    assert rel_close_fraction(sg, so, 1e-9) >= 0.99
+
+
+def test_no_bvh_render_matches_bvh_render(ctx):
+    o = orc.OracleScene(scenes.load("cornell_c1.json"), width=16, height=16)
+    a, _, _ = gpu_render(ctx, o, 0, 4, rtg.PRECISION_F64)
+    o2 = orc.OracleScene(scenes.load("cornell_c1.json"), width=16, height=16, use_bvh=False)
+    b, _, _ = gpu_render(ctx, o2, 0, 4, rtg.PRECISION_F64)
+    assert rel_close_fraction(a, b, 1e-9) >= 0.99
