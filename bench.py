@@ -1,0 +1,172 @@
+"""Benchmark: G rays/s of the MI355X path tracer on the 172,800-triangle
+torus workload (1024x1024, 64 spp, depth 6, seed 7), with the roofline of
+the closest-hit traversal kernel and the reference CPU renderer timed on this
+host beside it.
+
+A step is one full render of the workload through the product path: the
+scene compiled by the C++ host (JSON -> Scene::compile -> binned-SAH BVH),
+resident on the device, rendered with the reference's progressive pass
+schedule (1, 1, 2, 4, 8, 16, 32 samples) into a cleared accumulation buffer.
+Rays are counted by the reference's rules (closest-hit rays + shadow rays).
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload torus_c2]
+"""
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "ray-tracer_amd")
+
+WORKLOADS = {
+    "torus_c2": "torus_c2.json",
+    "cornell_c1": "cornell_c1.json",
+    "sphere_field_c3": "sphere_field_c3.json",
+}
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# Algorithmic bytes of the FP32 closest-hit kernel: 32 B per node box tested
+# (a 64-byte record holds both children), 48 B per triangle test, 28 B of ray
+# read (queue slot + origin + direction) and 20 B of hit written per ray.
+NODE_B, TRI_B, RAY_IN_B, HIT_OUT_B = 32, 48, 28, 20
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location("ray_tracer_amd", os.path.join(PKG, "__init__.py"),
+                                                  submodule_search_locations=[PKG])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["ray_tracer_amd"] = mod
+    spec.loader.exec_module(mod)
+    from ray_tracer_amd import rtg
+    return rtg
+
+
+def render_once(rtg, ctx, dev, acc, cfg, spp, max_pass=32):
+    """One progressive render of `spp` samples (renderer.cpp:66-69)."""
+    acc.clear()
+    done, rays = 0, 0
+    while done < spp:
+        k = min(spp - done, max(1, done), max_pass)
+        st, _ = ctx.render_pass(dev, cfg, done, done + k, acc)
+        rays += st.rays + st.shadow_rays
+        done += k
+    return rays
+
+
+def cpu_baseline(scene_text, width, height, target_s=12.0):
+    """The reference renderer (oracle/_ref, built from the reference's own
+    sources) on this host's cores, on a bounded sample of the workload:
+    the full image at a reduced sample count chosen for ~target_s seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    threads = max(1, min(16, os.cpu_count() or 1))
+    if orc.have_ref():
+        kind = "reference"
+
+        def run(spp):
+            r = orc.RefScene(scene_text, width=width, height=height, spp=spp)
+            stats = (C.c_uint64 * 5)()
+            ms = C.c_double()
+            orc.ref().ref_render_full(r.h, threads, 1, 32, None, stats, C.byref(ms))
+            return (stats[1] + stats[2]), ms.value / 1e3
+    else:
+        kind = "port"
+
+        def run(spp):
+            o = orc.OracleScene(json.loads(scene_text), width=width, height=height)
+            t = time.perf_counter()
+            _, _, st = o.render(0, spp, threads=threads)
+            return st["rays"] + st["shadow_rays"], time.perf_counter() - t
+    rays1, s1 = run(1)
+    spp = int(max(1, min(64, round(target_s / max(s1, 1e-3)))))
+    rays, secs = run(spp) if spp > 1 else (rays1, s1)
+    return {"value": rays / secs / 1e9, "unit": "G rays/s", "cores": threads, "kind": kind,
+            "sample": f"{width}x{height} at {spp} spp (of 64), progressive passes, {rays} rays in {secs:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="torus_c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build renders on one GPU; multi-GPU runs are not part of it "
+              "(the sample range would shard across GPUs with one FP32 sum+count reduction)", file=sys.stderr)
+        sys.exit(2)
+    rtg = load_pkg()
+    with open(os.path.join(PKG, "scenes", WORKLOADS[args.workload])) as f:
+        text = f.read()
+    t0 = time.perf_counter()
+    hs = rtg.HostScene(text)
+    prep_ms = (time.perf_counter() - t0) * 1e3
+    info = hs.info()
+    precision = rtg.PRECISION_F64 if args.precision == "f64" else rtg.PRECISION_F32
+    cfg = hs.settings(precision)
+    spp = int(info["spp"])
+    ctx = rtg.Context(0)
+    dev = ctx.scene(hs.desc)
+    acc = ctx.accum(cfg.width, cfg.height)
+    for _ in range(args.warmup):
+        render_once(rtg, ctx, dev, acc, cfg, spp)
+    ctx.synchronize()
+    rtg.set_profiling(ctx, True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    rays = 0
+    for _ in range(args.steps):
+        rays += render_once(rtg, ctx, dev, acc, cfg, spp)
+    ctx.synchronize()
+    secs = time.perf_counter() - t0
+    prof = rtg.get_profile(ctx).as_dict()
+    rtg.set_profiling(ctx, False)
+    value = rays / secs / 1e9
+
+    ext = prof["extend"]
+    n_launch = max(1, ext["launches"])
+    bytes_total = (NODE_B * prof["closest_visits"] + TRI_B * prof["closest_tests"] +
+                   (RAY_IN_B + HIT_OUT_B) * prof["closest_rays"])
+    per_launch = bytes_total / n_launch
+    avg_ms = ext["ms"] / n_launch
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_extend_{args.workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"kernel": "k_extend (FP32 closest-hit traversal)", "bound": "hbm", "achieved": achieved,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches": ext["launches"],
+                "node_visits_per_ray": prof["closest_visits"] / max(1, prof["closest_rays"]),
+                "tests_per_ray": prof["closest_tests"] / max(1, prof["closest_rays"])}
+    kernel_ms = {k: round(v["ms"] / args.steps, 3) for k, v in prof.items() if isinstance(v, dict)}
+    dev.release()
+    acc.release()
+    ctx.close()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(text, cfg.width, cfg.height)
+    line = {
+        "metric": "G rays/s + % HBM roofline; served p50/p99 latency vs plain GPU loop",
+        "value": value, "unit": "G rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": secs / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic (procedural torus scene, seeded RNG)",
+        "config": {"workload": f"{args.workload}: {info['triangles']} triangles + {info['spheres']} sphere + "
+                               f"{info['planes']} plane, {cfg.width}x{cfg.height}, {spp} spp, depth {cfg.max_depth}, "
+                               f"seed {cfg.seed}, progressive passes",
+                   "bvh_nodes": info["nodes"], "bvh_depth": info["max_depth"], "scene_prep_ms": round(prep_ms, 1),
+                   "rays_per_step": rays // args.steps, "kernel_ms_per_step": kernel_ms},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -237,6 +237,24 @@ rtg_status rtg_trace_host(rtg_context* ctx, const rtg_scene* scene, const double
 
 rtg_status rtg_synchronize(rtg_context* ctx);
 
+/* ------------------------------------------------------------------------ */
+/* Per-kernel device timing of the render passes and traces issued since
+ * profiling was switched on (HIP events around every launch, on the
+ * context's stream). kind: 0 generate, 1 extend (closest hit), 2 shade,
+ * 3 shadow (any hit), 4 accumulate, 5 direct/normals sample, 6 trace. */
+enum { RTG_K_GENERATE = 0, RTG_K_EXTEND, RTG_K_SHADE, RTG_K_SHADOW, RTG_K_ACCUMULATE, RTG_K_DIRECT, RTG_K_TRACE,
+       RTG_K_COUNT };
+typedef struct rtg_profile {
+  uint64_t launches[RTG_K_COUNT];
+  double ms[RTG_K_COUNT];
+  /* work done by the closest-hit kernels (extend, trace, direct) and by the
+   * any-hit kernel (shadow): rays, node boxes tested, primitive tests */
+  uint64_t closest_rays, closest_visits, closest_tests;
+  uint64_t shadow_rays, shadow_visits, shadow_tests;
+} rtg_profile;
+rtg_status rtg_set_profiling(rtg_context* ctx, int32_t on); /* on: clears the totals */
+rtg_status rtg_get_profile(rtg_context* ctx, rtg_profile* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,13 @@ struct rtg_context {
   PathBuffers<float> pb32{};
   PathBuffers<double> pb64{};
   uint32_t cap[2] = {0, 0};
+  // Profiling: an event pair per launch, resolved when the call completes.
+  bool profiling = false;
+  std::vector<hipEvent_t> events;
+  size_t events_used = 0;
+  std::vector<int> event_kind;  // kind of each begin event
+  rtg_profile profile{};
+  LaunchHooks hooks{};
 };
 
 struct rtg_scene {
@@ -143,11 +150,66 @@ rtg_status validate_desc(rtg_context* ctx, const rtg_scene_desc* d) {
   return RTG_OK;
 }
 
+void profile_mark(void* user, int kind, bool begin) {
+  auto* ctx = static_cast<rtg_context*>(user);
+  if (ctx->events_used == ctx->events.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    ctx->events.push_back(e);
+  }
+  if (begin) {
+    if (ctx->event_kind.size() < ctx->events_used + 1) ctx->event_kind.resize(ctx->events_used + 1);
+    ctx->event_kind[ctx->events_used] = kind;
+  }
+  hipEventRecord(ctx->events[ctx->events_used++], ctx->stream);
+}
+
+// After the stream has drained: add every begin/end pair to the totals.
+void profile_resolve(rtg_context* ctx, const unsigned long long* h) {
+  if (!ctx->profiling) return;
+  for (size_t i = 0; i + 1 < ctx->events_used; i += 2) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ctx->events[i], ctx->events[i + 1]) == hipSuccess) {
+      const int k = ctx->event_kind[i];
+      ctx->profile.ms[k] += ms;
+      ctx->profile.launches[k]++;
+    }
+  }
+  ctx->events_used = 0;
+  ctx->profile.closest_rays += h[kStRays];
+  ctx->profile.closest_visits += h[kStVisits];
+  ctx->profile.closest_tests += h[kStTests];
+  ctx->profile.shadow_rays += h[kStShadow];
+  ctx->profile.shadow_visits += h[kStSVisits];
+  ctx->profile.shadow_tests += h[kStSTests];
+}
+
+const LaunchHooks* hooks_of(rtg_context* ctx) {
+  if (!ctx->profiling) return nullptr;
+  ctx->hooks.user = ctx;
+  ctx->hooks.mark = profile_mark;
+  return &ctx->hooks;
+}
+
 }  // namespace
 
 extern "C" {
 
 int rtg_abi_version(void) { return RTG_ABI_VERSION; }
+
+rtg_status rtg_set_profiling(rtg_context* ctx, int32_t on) {
+  if (!ctx) return fail(nullptr, RTG_ERR_INVALID_ARG, "rtg_set_profiling: NULL context");
+  ctx->profiling = on != 0;
+  ctx->profile = rtg_profile{};
+  ctx->events_used = 0;
+  return RTG_OK;
+}
+
+rtg_status rtg_get_profile(rtg_context* ctx, rtg_profile* out) {
+  if (!ctx || !out) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_get_profile: NULL argument");
+  *out = ctx->profile;
+  return RTG_OK;
+}
 
 const char* rtg_last_error(const rtg_context* ctx) { return ctx ? ctx->err.c_str() : g_error.c_str(); }
 
@@ -181,6 +243,7 @@ rtg_status rtg_close(rtg_context* ctx) {
   if (ctx->stats) hipFree(ctx->stats);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  for (hipEvent_t e : ctx->events) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return RTG_OK;
@@ -551,14 +614,15 @@ uint32_t chunk_paths_default() {
 void read_stats(rtg_context* ctx, rtg_stats* out, float ms) {
   unsigned long long h[kStCount] = {0};
   hipMemcpy(h, ctx->stats, sizeof h, hipMemcpyDeviceToHost);
+  profile_resolve(ctx, h);
   if (!out) return;
   out->camera_samples = h[kStCamera];
   out->rays = h[kStRays];
   out->shadow_rays = h[kStShadow];
   out->rr_terminated = h[kStRR];
   out->invalid_samples = h[kStInvalid];
-  out->node_visits = h[kStVisits];
-  out->prim_tests = h[kStTests];
+  out->node_visits = h[kStVisits] + h[kStSVisits];
+  out->prim_tests = h[kStTests] + h[kStSTests];
   out->elapsed_ms = ms;
 }
 
@@ -616,8 +680,8 @@ rtg_status rtg_render_pass(rtg_context* ctx, const rtg_scene* scene, const rtg_r
       P.seed = cfg->seed;
       P.acc_sum = acc->sum;
       P.acc_count = acc->count;
-      if (f64) run_chunk(scene->dev, ctx->pb64, P, scene->stack, ctx->stream);
-      else run_chunk(scene->dev, ctx->pb32, P, scene->stack, ctx->stream);
+      if (f64) run_chunk(scene->dev, ctx->pb64, P, scene->stack, ctx->stream, hooks_of(ctx));
+      else run_chunk(scene->dev, ctx->pb32, P, scene->stack, ctx->stream, hooks_of(ctx));
       HIP_TRY(ctx, hipGetLastError());
       if (cancel) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // poll between chunks
     }
@@ -682,10 +746,10 @@ rtg_status rtg_trace(rtg_context* ctx, const rtg_scene* scene, const rtg_buffer*
   if (n) {
     if (opts->precision == RTG_PRECISION_F64)
       run_trace64(scene->dev, static_cast<const double*>(rays->ptr), n, opts->mode, opts->use_bvh,
-                  static_cast<rtg_hit*>(hits->ptr), ctx->stats, ctx->stream);
+                  static_cast<rtg_hit*>(hits->ptr), ctx->stats, ctx->stream, hooks_of(ctx));
     else
       run_trace32(scene->dev, static_cast<const double*>(rays->ptr), n, opts->mode, opts->use_bvh,
-                  static_cast<rtg_hit*>(hits->ptr), ctx->stats, scene->stack, ctx->stream);
+                  static_cast<rtg_hit*>(hits->ptr), ctx->stats, scene->stack, ctx->stream, hooks_of(ctx));
     HIP_TRY(ctx, hipGetLastError());
   }
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));

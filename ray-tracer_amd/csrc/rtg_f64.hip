@@ -42,26 +42,27 @@ __global__ void k_resolve(const double* sum, const uint32_t* count, int n, int o
 }
 }  // namespace
 
-void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t, hipStream_t s) {
+void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t, hipStream_t s,
+               const LaunchHooks* h) {
   const uint32_t g = grid_of(P.n_paths);
   if (P.integrator != RTG_INTEGRATOR_PATH) {
-    hipLaunchKernelGGL((k_direct<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P);
+    RTG_LAUNCH(h, RTG_K_DIRECT, (k_direct<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P);
   } else {
-    hipLaunchKernelGGL(k_generate<double>, dim3(g), dim3(kBlock), 0, s, S, B, P);
+    RTG_LAUNCH(h, RTG_K_GENERATE, k_generate<double>, dim3(g), dim3(kBlock), 0, s, S, B, P);
     const bool nee = P.light_sampling && S.n_lights > 0;
     for (int b = 0; b < P.max_depth; ++b) {
-      hipLaunchKernelGGL((k_extend<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
-      hipLaunchKernelGGL(k_shade<double>, dim3(g), dim3(kBlock), 0, s, S, B, P, b);
-      if (nee) hipLaunchKernelGGL((k_shadow<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      RTG_LAUNCH(h, RTG_K_SHADE, k_shade<double>, dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      if (nee) RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
     }
   }
-  hipLaunchKernelGGL(k_accumulate<double>, dim3(grid_of(P.n_pix)), dim3(kBlock), 0, s, B, P);
+  RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_accumulate<double>, dim3(grid_of(P.n_pix)), dim3(kBlock), 0, s, B, P);
 }
 
 void run_trace64(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,
-                 unsigned long long* stats, hipStream_t s) {
-  hipLaunchKernelGGL((k_trace<double, 0>), dim3(grid_of(n)), dim3(kBlock), 0, s, S, rays, n, mode, use_bvh, hits,
-                     stats);
+                 unsigned long long* stats, hipStream_t s, const LaunchHooks* h) {
+  RTG_LAUNCH(h, RTG_K_TRACE, (k_trace<double, 0>), dim3(grid_of(n)), dim3(kBlock), 0, s, S, rays, n, mode, use_bvh,
+             hits, stats);
 }
 
 void run_resolve(const double* sum, const uint32_t* count, int w, int h, int op, double exposure, uint8_t* rgb,

@@ -23,7 +23,7 @@ namespace rtg {
 
 constexpr int kBlock = 128;
 
-enum StatIdx { kStCamera = 0, kStRays, kStShadow, kStRR, kStInvalid, kStVisits, kStTests, kStCount };
+enum StatIdx { kStCamera = 0, kStRays, kStShadow, kStRR, kStInvalid, kStVisits, kStTests, kStSVisits, kStSTests, kStCount };
 
 template <class T>
 struct PathBuffers {
@@ -320,8 +320,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, PathBuffers<T> B,
       B.ly[pid] = B.ly[pid] + B.scy[i];
       B.lz[pid] = B.lz[pid] + B.scz[i];
     }
-    st[kStVisits] = c.visits;
-    st[kStTests] = c.tests;
+    st[kStSVisits] = c.visits;
+    st[kStSTests] = c.tests;
   }
   flush_stats(B.stats, st, kStCount);
 }
@@ -435,8 +435,8 @@ __device__ V3<T> direct_sample(const DevScene& S, const ChunkParams& P, V3<T> o,
           ++st[kStShadow];
           Counters c2{0, 0};
           if (!ModeOps<T>::occluded(S, P.use_bvh != 0, req.o, req.d, req.tmax, stk, c2)) f.L = f.L + req.f_rop;
-          st[kStVisits] += c2.visits;
-          st[kStTests] += c2.tests;
+          st[kStSVisits] += c2.visits;
+          st[kStSTests] += c2.tests;
         }
       }
       ret = f.L;

@@ -10,15 +10,30 @@ namespace rtg {
 // Stack classes of the FP32 traversal kernels (entries per lane in LDS).
 constexpr uint32_t kStackClasses[3] = {20, 32, 64};
 
+// Optional per-launch hook (profiling): called before and after every
+// kernel launch with the RTG_K_* kind.
+struct LaunchHooks {
+  void* user;
+  void (*mark)(void* user, int kind, bool begin);
+};
+#define RTG_LAUNCH(hooks, kind, ...)                                \
+  do {                                                              \
+    if (hooks) (hooks)->mark((hooks)->user, (kind), true);          \
+    hipLaunchKernelGGL(__VA_ARGS__);                                \
+    if (hooks) (hooks)->mark((hooks)->user, (kind), false);         \
+  } while (0)
+
 // Runs one chunk of a render pass: generate, max_depth x (extend, shade,
 // shadow), accumulate; or the per-sample direct/normals kernel.
-void run_chunk(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P, uint32_t stack, hipStream_t s);
-void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t stack, hipStream_t s);
+void run_chunk(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
+               const LaunchHooks* hooks);
+void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
+               const LaunchHooks* hooks);
 
 void run_trace32(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,
-                 unsigned long long* stats, uint32_t stack, hipStream_t s);
+                 unsigned long long* stats, uint32_t stack, hipStream_t s, const LaunchHooks* hooks);
 void run_trace64(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,
-                 unsigned long long* stats, hipStream_t s);
+                 unsigned long long* stats, hipStream_t s, const LaunchHooks* hooks);
 
 // Tone map + sRGB8 (tonemap.cpp:14-53), FP64 like the reference.
 void run_resolve(const double* sum, const uint32_t* count, int w, int h, int op, double exposure, uint8_t* rgb,

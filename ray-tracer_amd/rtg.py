@@ -159,7 +159,7 @@ def rtg_symbols():
             "rtg_scene_release", "rtg_scene_bytes", "rtg_accum_create", "rtg_accum_release", "rtg_accum_clear",
             "rtg_accum_read", "rtg_accum_min_samples", "rtg_render_pass", "rtg_resolve", "rtg_buffer_create",
             "rtg_buffer_release", "rtg_buffer_write", "rtg_buffer_read", "rtg_trace", "rtg_trace_host",
-            "rtg_synchronize"]
+            "rtg_synchronize", "rtg_set_profiling", "rtg_get_profile"]
 
 
 def _ptr(a):
@@ -299,3 +299,174 @@ class HostArrays:
         d.camera = camera
         d.background = background
         self.desc = d
+
+
+# ---------------------------------------------------------------------------
+# Host library (include/rt_host.h, librthost.so)
+
+class Overrides(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32),
+                ("rr_min_depth", C.c_int32), ("integrator", C.c_int32), ("light_sampling", C.c_int32),
+                ("use_bvh", C.c_int32), ("seed", C.c_int64), ("exposure", C.c_double)]
+
+
+class HostRenderStats(C.Structure):
+    _fields_ = [("camera_samples", C.c_uint64), ("rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("rr_terminated", C.c_uint64), ("invalid_samples", C.c_uint64), ("node_visits", C.c_uint64),
+                ("prim_tests", C.c_uint64), ("passes", C.c_uint64), ("elapsed_ms", C.c_double),
+                ("device_ms", C.c_double), ("spp_done", C.c_uint32), ("cancelled", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_rth = None
+
+
+def load_host():
+    """Loads librthost.so (which links librtg.so)."""
+    global _rth
+    if _rth is not None:
+        return _rth
+    path = _lib_path("librthost.so")
+    if not os.path.exists(path):
+        raise RtgError(f"{path} is missing: build it first")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    sig = {
+        "rt_validate": (C.c_char_p, [C.c_char_p]),
+        "rt_message": (C.c_char_p, []),
+        "rt_threads": (C.c_int, []),
+        "rt_running": (C.c_int, []),
+        "rt_cancel": (None, []),
+        "rt_start": (C.c_int, [C.c_int32, C.c_char_p, C.c_int32, _VP, _VP, _VP]),
+        "rt_wait": (None, []),
+        "rth_scene_load": (C.c_int, [C.c_char_p, P(Overrides), P(_VP)]),
+        "rth_scene_free": (None, [_VP]),
+        "rth_last_error": (C.c_char_p, []),
+        "rth_scene_desc": (P(SceneDesc), [_VP]),
+        "rth_scene_settings": (None, [_VP, C.c_int32, P(RenderSettings)]),
+        "rth_scene_info": (None, [_VP, P(C.c_uint64)]),
+        "rth_scene_bvh_build_ms": (C.c_double, [_VP]),
+        "rth_dump_scene": (C.c_int64, [C.c_char_p, C.c_char_p, C.c_uint64]),
+        "rth_render": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _VP, _VP, _VP,
+                                 P(HostRenderStats)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype, f.argtypes = res, args
+    _rth = lib
+    return lib
+
+
+def host_symbols():
+    return ["rt_validate", "rt_message", "rt_threads", "rt_running", "rt_cancel", "rt_start", "rt_wait",
+            "rth_scene_load", "rth_scene_free", "rth_last_error", "rth_scene_desc", "rth_scene_settings",
+            "rth_scene_info", "rth_scene_bvh_build_ms", "rth_dump_scene", "rth_render"]
+
+
+FRAME_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.POINTER(C.c_uint8), C.c_void_p)
+DONE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_void_p)
+
+
+class HostScene:
+    """A scene parsed, validated and compiled by the C++ host (Scene::compile
+    + BVH build), viewable as an rtg SceneDesc."""
+
+    def __init__(self, text, **ov):
+        import json as _json
+        lib = load_host()
+        if not isinstance(text, str):
+            text = _json.dumps(text)
+        o = Overrides(-1, -1, -1, -1, -1, -1, -1, -1, -1, 0.0)
+        for k, v in ov.items():
+            if v is None:
+                continue
+            if k == "integrator":
+                v = INTEGRATORS[v]
+            if isinstance(v, bool):
+                v = int(v)
+            setattr(o, k, v)
+        h = C.c_void_p()
+        if lib.rth_scene_load(text.encode(), C.byref(o), C.byref(h)) != 0:
+            raise RtgError(lib.rth_last_error().decode())
+        self.lib, self.h = lib, h
+        self.desc = lib.rth_scene_desc(h).contents
+
+    def settings(self, precision=PRECISION_F32):
+        s = RenderSettings()
+        self.lib.rth_scene_settings(self.h, precision, C.byref(s))
+        return s
+
+    def info(self):
+        a = (C.c_uint64 * 10)()
+        self.lib.rth_scene_info(self.h, a)
+        keys = ["triangles", "spheres", "planes", "degenerate_skipped", "lights", "nodes", "leaves", "max_depth",
+                "spp", "tone_map"]
+        d = dict(zip(keys, list(a)))
+        d["bvh_build_ms"] = self.lib.rth_scene_bvh_build_ms(self.h)
+        return d
+
+    def render(self, precision=PRECISION_F32, progressive=True, max_pass=32, device=0):
+        cfg = self.settings(precision)
+        s = np.zeros((cfg.height, cfg.width, 3))
+        c = np.zeros((cfg.height, cfg.width), np.uint32)
+        rgb = np.zeros((cfg.height, cfg.width, 3), np.uint8)
+        st = HostRenderStats()
+        if self.lib.rth_render(self.h, device, precision, 1 if progressive else 0, max_pass, _ptr(s), _ptr(c),
+                               _ptr(rgb), C.byref(st)) != 0:
+            raise RtgError(self.lib.rth_last_error().decode())
+        return s, c, rgb, st.as_dict()
+
+    def close(self):
+        if self.h:
+            self.lib.rth_scene_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dump_scene(text):
+    lib = load_host()
+    buf = C.create_string_buffer(1 << 22)
+    n = lib.rth_dump_scene(text.encode(), buf, len(buf))
+    if n <= 0:
+        raise RtgError(lib.rth_last_error().decode())
+    return buf.value.decode()
+
+
+def validate(text):
+    return load_host().rt_validate(text.encode()).decode()
+
+
+K_NAMES = ["generate", "extend", "shade", "shadow", "accumulate", "direct", "trace"]
+
+
+class Profile(C.Structure):
+    _fields_ = [("launches", C.c_uint64 * 7), ("ms", C.c_double * 7),
+                ("closest_rays", C.c_uint64), ("closest_visits", C.c_uint64), ("closest_tests", C.c_uint64),
+                ("shadow_rays", C.c_uint64), ("shadow_visits", C.c_uint64), ("shadow_tests", C.c_uint64)]
+
+    def as_dict(self):
+        d = {k: {"launches": int(self.launches[i]), "ms": float(self.ms[i])} for i, k in enumerate(K_NAMES)}
+        for k in ("closest_rays", "closest_visits", "closest_tests", "shadow_rays", "shadow_visits", "shadow_tests"):
+            d[k] = int(getattr(self, k))
+        return d
+
+
+def set_profiling(ctx: "Context", on: bool):
+    lib = ctx.lib
+    lib.rtg_set_profiling.restype, lib.rtg_set_profiling.argtypes = C.c_int32, [_VP, C.c_int32]
+    ctx._check(lib.rtg_set_profiling(ctx.h, 1 if on else 0))
+
+
+def get_profile(ctx: "Context") -> Profile:
+    lib = ctx.lib
+    lib.rtg_get_profile.restype, lib.rtg_get_profile.argtypes = C.c_int32, [_VP, C.POINTER(Profile)]
+    p = Profile()
+    ctx._check(lib.rtg_get_profile(ctx.h, C.byref(p)))
+    return p
