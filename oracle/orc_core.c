@@ -615,3 +615,13 @@ int orc__prim_hit_t(const rtg_primitive* p, const double o[3], const double d[3]
 }
 uint32_t orc__rng_below(orc_rng* r, uint32_t n) { return rng_below(r, n); }
 double orc__rng_uniform_range(orc_rng* r, double lo, double hi) { return rng_uniform_range(r, lo, hi); }
+
+/* AABB::hit with inv = 1/d (aabb.hpp:43-52), for the slab known answers. */
+int orc_box_hit(const double lo[3], const double hi[3], const double o[3], const double d[3], double tmin,
+                double tmax) {
+  rtg_bvh_node n;
+  memset(&n, 0, sizeof n);
+  n.lo.x = lo[0]; n.lo.y = lo[1]; n.lo.z = lo[2];
+  n.hi.x = hi[0]; n.hi.y = hi[1]; n.hi.z = hi[2];
+  return box_hit(&n, vload(o), V(1.0 / d[0], 1.0 / d[1], 1.0 / d[2]), tmin, tmax);
+}

@@ -47,6 +47,8 @@ int orc_bvh_build(const double* bounds, uint32_t n, int max_leaf, rtg_bvh_node* 
 int orc_bvh_validate(const rtg_bvh_node* nodes, uint32_t n_nodes, const uint32_t* prim_indices,
                      const double* bounds, uint32_t n);
 void orc_prim_bounds(const rtg_primitive* p, double out[6]);
+int orc_box_hit(const double lo[3], const double hi[3], const double o[3], const double d[3], double tmin,
+                double tmax);
 
 /* ---- Scene queries: src/core/scene.cpp:139-234 ---- */
 typedef struct orc_hit {

@@ -392,6 +392,7 @@ class HostScene:
             raise RtgError(lib.rth_last_error().decode())
         self.lib, self.h = lib, h
         self.desc = lib.rth_scene_desc(h).contents
+        self.desc._owner = self  # the arrays live in the host scene
 
     def settings(self, precision=PRECISION_F32):
         s = RenderSettings()

@@ -1,0 +1,178 @@
+"""Writes the golden fixtures of tests/golden/ from the REFERENCE renderer
+itself (oracle/_ref/librtref.so, compiled from the reference's sources by
+oracle/Makefile). Run in the container that has the reference checkout:
+
+    make -C oracle && python tests/golden/make_golden.py
+
+Each fixture is data only: inputs and the reference's outputs.
+"""
+import ctypes as C
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import orc  # noqa: E402
+import scenes  # noqa: E402
+
+OUT = os.path.join(HERE, "reference_vectors.npz")
+
+
+def sha(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def ray_set(seed, n, extent, axis_every=50):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(-extent, extent, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    for i in range(0, n, axis_every):
+        d[i] = (0.0, 0.0, 1.0 if i % 100 else -1.0)
+    r = np.zeros((n, 8))
+    r[:, :3], r[:, 3:6], r[:, 6], r[:, 7] = o, d, 1e-6, np.inf
+    return r
+
+
+def main():
+    assert orc.have_ref(), "oracle/_ref/librtref.so missing: run make -C oracle in the reference container"
+    L = orc.ref()
+    out, meta = {}, {}
+
+    # 1. RNG streams: first 8 outputs of Rng::for_sample on a grid.
+    grid = np.array([(s, p, k) for s in (0, 1, 7, 99, 2**40 + 3) for p in (0, 1, 12345, 2**32 - 1)
+                     for k in (0, 1, 63, 100000)], dtype=np.uint64)
+    streams = np.zeros((len(grid), 8), np.uint32)
+    for i, (s, p, k) in enumerate(grid):
+        buf = (C.c_uint32 * 8)()
+        L.ref_rng_stream(int(s), int(p), int(k), buf, 8)
+        streams[i] = list(buf)
+    out["rng_grid"], out["rng_streams"] = grid, streams
+
+    # 2. Compiled scenes + BVHs: exact bytes for small scenes, hashes for big ones.
+    names = ["cornell_box.json", "first_light.json", "materials.json", "sphere_field.json", "soft_shadows.json",
+             "mesh_bvh.json"]
+    for name in names:
+        r = orc.RefScene(json.dumps(scenes.load_asset(name)))
+        a = orc.arrays(r.desc)
+        key = "asset_" + name[:-5]
+        meta[key] = {k: sha(v) for k, v in a.items()}
+        meta[key]["info"] = r.info()
+        if r.desc.n_prims < 2000:
+            for k, v in a.items():
+                out[f"{key}_{k}"] = v
+    for name in ["cornell_c1.json", "torus_c2.json"]:
+        r = orc.RefScene(json.dumps(scenes.load(name)))
+        a = orc.arrays(r.desc)
+        key = "cfg_" + name[:-5]
+        meta[key] = {k: sha(v) for k, v in a.items()}
+        meta[key]["info"] = r.info()
+        if r.desc.n_prims < 2000:
+            for k, v in a.items():
+                out[f"{key}_{k}"] = v
+    for seed in (1, 2, 3):
+        r = orc.RefScene(json.dumps(scenes.random_scene_json(300, 600, seed, seed == 2)))
+        a = orc.arrays(r.desc)
+        meta[f"random_{seed}"] = {k: sha(v) for k, v in a.items()}
+        for k in ("nodes", "prim_indices"):
+            out[f"random_{seed}_{k}"] = a[k]
+    # The segments_v compile path agrees with Scene::compile where both apply.
+    soft = json.dumps(scenes.load_asset("soft_shadows.json"))
+    a1 = orc.arrays(orc.RefScene(soft).desc)
+    a2 = orc.arrays(orc.RefScene(soft, force_segments_v_path=True).desc)
+    assert all(np.array_equal(a1[k], a2[k]) for k in a1), "segments_v path differs from Scene::compile"
+
+    # 3. Closest / any hits of seeded rays (Scene::intersect / occluded).
+    for seed in (1, 2):
+        r = orc.RefScene(json.dumps(scenes.random_scene_json(300, 600, seed, seed == 2)))
+        rays = ray_set(seed * 100, 4096, 15.0)
+        h, nv, pt = r.trace(rays, 0, 1)
+        hb, _, _ = r.trace(rays, 0, 0)
+        r2 = rays.copy()
+        r2[:, 7] = 5.0
+        ho, _, _ = r.trace(r2, 1, 1)
+        out[f"hits_random_{seed}_rays"] = rays
+        out[f"hits_random_{seed}_closest"] = h.view(np.uint8).reshape(len(h), -1)
+        out[f"hits_random_{seed}_brute"] = hb.view(np.uint8).reshape(len(h), -1)
+        out[f"hits_random_{seed}_any5"] = ho.view(np.uint8).reshape(len(h), -1)
+        meta[f"hits_random_{seed}"] = {"node_visits": nv, "prim_tests": pt}
+    torus = orc.RefScene(json.dumps(scenes.load("torus_c2.json")))
+    rays = ray_set(7, 8192, 2.0, axis_every=97)
+    h, nv, pt = torus.trace(rays, 0, 1)
+    out["hits_torus_rays"] = rays
+    out["hits_torus_closest"] = h.view(np.uint8).reshape(len(h), -1)
+    meta["hits_torus"] = {"node_visits": nv, "prim_tests": pt}
+
+    # 4. Framebuffer sums (render_tile's sample loop through radiance()).
+    for name, w, h_, spp, ov in [("cornell_c1.json", 16, 16, 4, {}), ("torus_c2.json", 12, 12, 2, {})]:
+        r = orc.RefScene(json.dumps(scenes.load(name)), width=w, height=h_, **ov)
+        s, c, st = r.render_rows(0, spp, w, h_)
+        key = "fb_" + name[:-5]
+        out[key + "_sum"], out[key + "_count"] = s, c
+        meta[key] = {"stats": [int(x) for x in st], "w": w, "h": h_, "spp": spp}
+    for name, w, h_, spp, integ in [("first_light.json", 16, 12, 3, None), ("materials.json", 16, 8, 2, "direct"),
+                                    ("cornell_glass_metal.json", 12, 12, 2, None),
+                                    ("sphere_field.json", 16, 10, 2, None), ("mirrors.json", 12, 8, 2, "normals")]:
+        r = orc.RefScene(json.dumps(scenes.load_asset(name)), width=w, height=h_, integrator=integ)
+        s, c, st = r.render_rows(0, spp, w, h_)
+        key = "fb_" + name[:-5] + ("_" + integ if integ else "")
+        out[key + "_sum"], out[key + "_count"] = s, c
+        meta[key] = {"stats": [int(x) for x in st], "w": w, "h": h_, "spp": spp, "integrator": integ,
+                     "scene": name}
+
+    # 5. Camera rays (Camera::generate) of the thin-lens sphere field camera.
+    r = orc.RefScene(json.dumps(scenes.load_asset("sphere_field.json")))
+    px = np.array([[i, j, sx, sy] for i, j in ((0, 0), (399, 225), (799, 449)) for sx, sy in ((0, 0), (0.5, 0.25))],
+                  dtype=np.float64)
+    rays_out = np.zeros((len(px), 6))
+    L.ref_camera_rays(r.h, px.ctypes.data_as(C.c_void_p), len(px), 5, rays_out.ctypes.data_as(C.c_void_p))
+    cam = (C.c_double * 14)()
+    L.ref_scene_camera_desc(r.h, cam)
+    out["camera_px"], out["camera_rays"], out["camera_desc"] = px, rays_out, np.array(list(cam))
+
+    # 6. Tone map and sRGB encode.
+    vals = np.concatenate([np.linspace(-0.5, 1.5, 257), [0.0031308, 0.5, 1e9]])
+    out["srgb_in"] = vals
+    out["srgb_out"] = np.array([L.ref_linear_to_srgb8(float(v)) for v in vals], np.uint8)
+    tm = np.zeros((3, len(vals), 3))
+    for op in range(3):
+        for i, v in enumerate(vals):
+            a = (C.c_double * 3)(v, v * 0.5, v * 2)
+            b = (C.c_double * 3)()
+            L.ref_tone_map(a, op, 1.3, b)
+            tm[op, i] = list(b)
+    out["tonemap_out"] = tm
+
+    # 7. Scene JSON round trip (scene_to_json) of every shipped scene.
+    dumps = {}
+    for fn in sorted(os.listdir(scenes.ASSET_DIR)):
+        text = open(os.path.join(scenes.ASSET_DIR, fn)).read()
+        buf = C.create_string_buffer(1 << 22)
+        L.ref_dump_scene(text.encode(), buf, len(buf))
+        dumps[fn] = hashlib.sha256(buf.value).hexdigest()
+    meta["dump_scene_sha256"] = dumps
+
+    # 8. Validation messages of the reference for malformed scenes.
+    bad = json.load(open(os.path.join(HERE, "bad_scenes.json")))
+    msgs = []
+    for text in bad:
+        buf = C.create_string_buffer(4096)
+        L.ref_validate(text.encode(), buf, len(buf))
+        msgs.append(buf.value.decode())
+    meta["bad_scene_messages"] = msgs
+
+    np.savez_compressed(OUT, **out)
+    with open(os.path.join(HERE, "reference_meta.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+    print("wrote", OUT, os.path.getsize(OUT), "bytes;", len(out), "arrays")
+
+
+if __name__ == "__main__":
+    main()

@@ -273,6 +273,7 @@ class OracleScene:
         d.prim_indices, d.n_prim_indices = self.c.prim_indices, self.c.n_prim_indices
         d.camera, d.background = camera, b
         self.desc = d
+        d._owner = self  # the arrays live in the oracle's compiled scene
 
     def __del__(self):
         try:
@@ -332,6 +333,7 @@ class RefScene:
         if not self.h:
             raise RuntimeError(L.ref_last_error().decode())
         self.desc = L.ref_scene_desc(self.h).contents
+        self.desc._owner = self  # the arrays live in the C++ scene
 
     def __del__(self):
         try:
