@@ -1,5 +1,7 @@
 // rtg_f32.hip — FP32 instantiation of the path-tracing kernels (the
 // production path: float arithmetic, counter-hash RNG, LDS traversal stack).
+#include <algorithm>
+
 #include "rtg_launch.hpp"
 
 namespace rtg {
@@ -14,7 +16,7 @@ void chunk_impl(const DevScene& S, const PathBuffers<float>& B, const ChunkParam
   if (P.integrator != RTG_INTEGRATOR_PATH) {
     RTG_LAUNCH(h, RTG_K_DIRECT, (k_direct<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P);
   } else {
-    RTG_LAUNCH(h, RTG_K_GENERATE, k_generate<float>, dim3(g), dim3(kBlock), 0, s, S, B, P);
+    RTG_LAUNCH(h, RTG_K_GENERATE, k_generate<float>, dim3(grid_of(std::max<uint64_t>(P.n_paths, (uint64_t)(P.max_depth + 1) * kSubQueues))), dim3(kBlock), 0, s, S, B, P);
     const bool nee = P.light_sampling && S.n_lights > 0;
     for (int b = 0; b < P.max_depth; ++b) {
       RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);

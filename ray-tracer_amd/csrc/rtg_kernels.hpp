@@ -35,38 +35,107 @@ struct PathBuffers {
   uint32_t* hprim;                 // hit primitive (0xFFFFFFFF none)
   int32_t* hplane;                 // hit plane (-1 none)
   uint64_t *rs, *ri;               // PCG32 state/inc (FP64 mode)
-  uint32_t* q[2];                  // active queues
-  uint32_t* counts;                // [b] queue size at bounce b; [kMaxBounce + b] shadow queue size
-  uint32_t* sq_pid;                // shadow queue
+  uint32_t* q[2];                  // active queues (kSubQueues sub-queues of sub_cap each)
+  uint32_t* counts;                // sub-queue sizes: [(b)*kSubQueues + s] active, [(kMaxBounce+b)*kSubQueues + s] shadow
+  uint32_t* sq_pid;                // shadow queue (same sub-queue layout)
   T *sox, *soy, *soz, *sdx, *sdy, *sdz, *stm, *scx, *scy, *scz;
-  unsigned long long* stats;       // kStCount counters
-  uint32_t capacity;
+  unsigned long long* stats;       // kStatSlots slots of kStatStride counters
+  uint32_t capacity;               // paths per chunk
+  uint32_t sub_cap;                // entries per sub-queue
 };
 constexpr int kMaxBounce = 1024;
+// Queue appends and statistics are spread over independent counters (one
+// per sub-queue / slot, chosen by block index) so that no single address
+// takes an atomic from every block of a launch.
+constexpr int kSubQueues = 32;
+constexpr int kStatSlots = 256;
+constexpr int kStatStride = 16;  // counters per slot (128-byte lines)
+__host__ __device__ constexpr uint32_t sub_queue_cap(uint32_t capacity) {
+  return ((capacity + kBlock - 1) / kBlock + kSubQueues - 1) / kSubQueues * kBlock;
+}
 
 struct ChunkParams {
   uint32_t width, height;
   uint32_t p0, k, s0;  // pixels [p0, p0 + n_pix), samples [s0, s0 + k)
   uint32_t n_pix, n_paths;
   int32_t max_depth, rr_min_depth, integrator, light_sampling, use_bvh;
+  int32_t sample_major;  // path id layout: 0 pixel-major (samples of a pixel adjacent), 1 sample-major
   uint64_t seed;
   double* acc_sum;      // width*height*3
   uint32_t* acc_count;  // width*height
 };
 
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-  const unsigned long long mask = __ballot(pred);
-  if (mask == 0) return 0;
-  const uint32_t lane = __lane_id();
-  const int leader = __ffsll((long long)mask) - 1;
-  uint32_t base = 0;
-  if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader);
-  return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Block-level reduction of per-thread counters, one global atomic per block.
+// Compacted appends of the whole block to two queues at once (the active
+// queue and the shadow queue): wave ballots, an LDS total per block, one
+// global atomic per queue per block on the block's own sub-queue counter.
+// Must be called by every thread of the block. Returns -1 or the slot.
+struct AppendSlots {
+  int64_t a, b;
+};
+__device__ __forceinline__ AppendSlots block_append2(uint32_t* counts_a, uint32_t* counts_b, uint32_t sub_cap,
+                                                     bool pa, bool pb) {
+  __shared__ uint32_t s_wave[2][kBlock / 64];
+  __shared__ uint32_t s_base[2];
+  const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const unsigned long long ma = __ballot(pa), mb = __ballot(pb);
+  if (lane == 0) {
+    s_wave[0][wave] = (uint32_t)__popcll(ma);
+    s_wave[1][wave] = (uint32_t)__popcll(mb);
+  }
+  __syncthreads();
+  const uint32_t sub = blockIdx.x % kSubQueues;
+  if (threadIdx.x < 2) {
+    uint32_t tot = 0;
+    for (uint32_t w = 0; w < kBlock / 64; ++w) tot += s_wave[threadIdx.x][w];
+    uint32_t* c = (threadIdx.x == 0 ? counts_a : counts_b) + sub;
+    s_base[threadIdx.x] = tot ? atomicAdd(c, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t oa = s_base[0], ob = s_base[1];
+  for (uint32_t w = 0; w < wave; ++w) {
+    oa += s_wave[0][w];
+    ob += s_wave[1][w];
+  }
+  AppendSlots r;
+  r.a = pa ? (int64_t)sub * sub_cap + oa + lanes_below(ma) : -1;
+  r.b = pb ? (int64_t)sub * sub_cap + ob + lanes_below(mb) : -1;
+  return r;
+}
+
+// Maps dense index i of a queue to its storage slot: the block reads the
+// kSubQueues counts once into LDS as a prefix sum. Returns false past the end.
+struct QueueView {
+  uint32_t total;
+};
+__device__ __forceinline__ QueueView load_queue(const uint32_t* counts, uint32_t* pre) {
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int k = 0; k < kSubQueues; ++k) {
+      pre[k] = s;
+      s += counts[k];
+    }
+    pre[kSubQueues] = s;
+  }
+  __syncthreads();
+  return QueueView{pre[kSubQueues]};
+}
+__device__ __forceinline__ uint32_t queue_slot(const uint32_t* pre, uint32_t sub_cap, uint32_t i) {
+  int lo = 0, hi = kSubQueues - 1;  // last k with pre[k] <= i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return (uint32_t)lo * sub_cap + (i - pre[lo]);
+}
+
+// Block-level reduction of per-thread counters into one of kStatSlots
+// slots (the host sums the slots).
 __device__ __forceinline__ void flush_stats(unsigned long long* g, const uint32_t* v, int n) {
   __shared__ unsigned long long sh[kStCount];
   if (threadIdx.x < kStCount) sh[threadIdx.x] = 0;
@@ -74,7 +143,8 @@ __device__ __forceinline__ void flush_stats(unsigned long long* g, const uint32_
   for (int i = 0; i < n; ++i)
     if (v[i]) atomicAdd(&sh[i], (unsigned long long)v[i]);
   __syncthreads();
-  if (threadIdx.x < (unsigned)n && sh[threadIdx.x]) atomicAdd(&g[threadIdx.x], sh[threadIdx.x]);
+  if (threadIdx.x < (unsigned)n && sh[threadIdx.x])
+    atomicAdd(&g[(blockIdx.x % kStatSlots) * kStatStride + threadIdx.x], sh[threadIdx.x]);
 }
 
 template <class T>
@@ -159,18 +229,33 @@ __device__ __forceinline__ LdsStack lds_stack(uint32_t* ref, float* tn) {
 }
 
 // ---------------------------------------------------------------------------
-// generate: one camera ray per path (renderer.cpp:31-37).
+// Path id -> (pixel, sample). Pixel-major keeps the samples of one pixel
+// in adjacent lanes (their rays start together: the most coherent
+// traversal); sample-major puts neighbouring pixels of one sample together
+// (coalesced accumulation). Results do not depend on the layout.
+__device__ __forceinline__ uint32_t pid_pixel(const ChunkParams& P, uint32_t pid) {
+  return P.p0 + (P.sample_major ? pid % P.n_pix : pid / P.k);
+}
+__device__ __forceinline__ uint32_t pid_sample(const ChunkParams& P, uint32_t pid) {
+  return P.s0 + (P.sample_major ? pid / P.n_pix : pid % P.k);
+}
+__device__ __forceinline__ uint32_t pid_of(const ChunkParams& P, uint32_t j, uint32_t s) {
+  return P.sample_major ? s * P.n_pix + j : j * P.k + s;
+}
+
+// generate: one camera ray per path (renderer.cpp:31-37). The bounce-0
+// queue is the identity and is not stored.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_generate(DevScene S, PathBuffers<T> B, ChunkParams P) {
   const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pid == 0) {
-    B.counts[0] = P.n_paths;
-    for (int b = 1; b <= P.max_depth && b < kMaxBounce; ++b) B.counts[b] = 0;
-    for (int b = 0; b < P.max_depth && b < kMaxBounce; ++b) B.counts[kMaxBounce + b] = 0;
+  const uint32_t nz = (uint32_t)(P.max_depth + 1) * kSubQueues;
+  if (pid < nz) {
+    B.counts[pid] = pid == 0 ? P.n_paths : 0u;
+    B.counts[kMaxBounce * kSubQueues + pid] = 0u;
   }
   if (pid >= P.n_paths) return;
-  const uint32_t pixel = P.p0 + pid / P.k;
-  const uint32_t sample = P.s0 + pid % P.k;
+  const uint32_t pixel = pid_pixel(P, pid);
+  const uint32_t sample = pid_sample(P, pid);
   const int x = (int)(pixel % P.width), y = (int)(pixel / P.width);
   V3<T> o, d;
   if constexpr (sizeof(T) == 8) {
@@ -192,7 +277,12 @@ __global__ __launch_bounds__(kBlock) void k_generate(DevScene S, PathBuffers<T> 
   B.bx[pid] = 1; B.by[pid] = 1; B.bz[pid] = 1;
   B.lx[pid] = 0; B.ly[pid] = 0; B.lz[pid] = 0;
   B.spec[pid] = 1;
-  B.q[0][pid] = pid;
+}
+
+// Path id of active entry i at bounce b (identity at bounce 0).
+template <class T>
+__device__ __forceinline__ uint32_t active_pid(const PathBuffers<T>& B, const uint32_t* pre, int b, uint32_t i) {
+  return b == 0 ? i : B.q[b & 1][queue_slot(pre, B.sub_cap, i)];
 }
 
 // extend: closest hit for the active queue of bounce b.
@@ -200,11 +290,12 @@ template <class T, int STACK>
 __global__ __launch_bounds__(kBlock) void k_extend(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
   __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
   __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
+  __shared__ uint32_t pre[kSubQueues + 1];
+  const QueueView qv = load_queue(B.counts + (size_t)b * kSubQueues, pre);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t n = B.counts[b];
   uint32_t st[kStCount] = {0};
-  if (i < n) {
-    const uint32_t pid = B.q[b & 1][i];
+  if (i < qv.total) {
+    const uint32_t pid = active_pid(B, pre, b, i);
     const V3<T> o{B.ox[pid], B.oy[pid], B.oz[pid]}, d{B.dx[pid], B.dy[pid], B.dz[pid]};
     Counters c{0, 0};
     const auto h = ModeOps<T>::closest(S, P.use_bvh != 0, o, d, lds_stack<STACK>(sref, stn), c);
@@ -223,16 +314,16 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene S, PathBuffers<T> B,
 // shade: one bounce of path_trace (integrator.cpp:103-137) after the hit.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
+  __shared__ uint32_t pre[kSubQueues + 1];
+  const QueueView qv = load_queue(B.counts + (size_t)b * kSubQueues, pre);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t n = B.counts[b];
   uint32_t st[kStCount] = {0};
   bool alive = false, shadow = false;
   uint32_t pid = 0;
   ShadowReq<T> req;
-  V3<T> nro{0, 0, 0}, nrd{0, 0, 0};
-  if (i < n) {
-    pid = B.q[b & 1][i];
-    const uint32_t pixel = P.p0 + pid / P.k, sample = P.s0 + pid % P.k;
+  if (i < qv.total) {
+    pid = active_pid(B, pre, b, i);
+    const uint32_t pixel = pid_pixel(P, pid), sample = pid_sample(P, pid);
     const V3<T> o{B.ox[pid], B.oy[pid], B.oz[pid]}, d{B.dx[pid], B.dy[pid], B.dz[pid]};
     V3<T> beta{B.bx[pid], B.by[pid], B.bz[pid]};
     V3<T> L{B.lx[pid], B.ly[pid], B.lz[pid]};
@@ -271,27 +362,25 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, PathBuffers<T> B, 
             beta = beta * (T(1) / q);
           }
         }
-        if (live) {
-          nro = offset_ray_origin(rec.p, rec.gn, sr.dir);
-          nrd = sr.dir;
-          alive = b + 1 < P.max_depth;
+        if (live && b + 1 < P.max_depth) {
+          const V3<T> nro = offset_ray_origin(rec.p, rec.gn, sr.dir);
+          alive = true;
           B.bx[pid] = beta.x; B.by[pid] = beta.y; B.bz[pid] = beta.z;
           B.spec[pid] = sr.specular ? 1 : 0;
+          B.ox[pid] = nro.x; B.oy[pid] = nro.y; B.oz[pid] = nro.z;
+          B.dx[pid] = sr.dir.x; B.dy[pid] = sr.dir.y; B.dz[pid] = sr.dir.z;
         }
       }
       DrawSrc<T>::store(B, pid, rng);
     }
     B.lx[pid] = L.x; B.ly[pid] = L.y; B.lz[pid] = L.z;
-    if (alive) {
-      B.ox[pid] = nro.x; B.oy[pid] = nro.y; B.oz[pid] = nro.z;
-      B.dx[pid] = nrd.x; B.dy[pid] = nrd.y; B.dz[pid] = nrd.z;
-    }
   }
-  // Converged: compacted appends.
-  const uint32_t qi = wave_append(&B.counts[b + 1], alive);
-  if (alive) B.q[(b + 1) & 1][qi] = pid;
-  const uint32_t si = wave_append(&B.counts[kMaxBounce + b], shadow);
+  // Converged: compacted appends to the next active queue and the shadow queue.
+  const AppendSlots a = block_append2(B.counts + (size_t)(b + 1) * kSubQueues,
+                                      B.counts + (size_t)(kMaxBounce + b) * kSubQueues, B.sub_cap, alive, shadow);
+  if (alive) B.q[(b + 1) & 1][a.a] = pid;
   if (shadow) {
+    const int64_t si = a.b;
     B.sq_pid[si] = pid;
     B.sox[si] = req.o.x; B.soy[si] = req.o.y; B.soz[si] = req.o.z;
     B.sdx[si] = req.d.x; B.sdy[si] = req.d.y; B.sdz[si] = req.d.z;
@@ -307,18 +396,20 @@ template <class T, int STACK>
 __global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
   __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
   __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
+  __shared__ uint32_t pre[kSubQueues + 1];
+  const QueueView qv = load_queue(B.counts + (size_t)(kMaxBounce + b) * kSubQueues, pre);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t n = B.counts[kMaxBounce + b];
   uint32_t st[kStCount] = {0};
-  if (i < n) {
-    const uint32_t pid = B.sq_pid[i];
-    const V3<T> o{B.sox[i], B.soy[i], B.soz[i]}, d{B.sdx[i], B.sdy[i], B.sdz[i]};
+  if (i < qv.total) {
+    const uint32_t k = queue_slot(pre, B.sub_cap, i);
+    const uint32_t pid = B.sq_pid[k];
+    const V3<T> o{B.sox[k], B.soy[k], B.soz[k]}, d{B.sdx[k], B.sdy[k], B.sdz[k]};
     Counters c{0, 0};
-    const bool occ = ModeOps<T>::occluded(S, P.use_bvh != 0, o, d, B.stm[i], lds_stack<STACK>(sref, stn), c);
+    const bool occ = ModeOps<T>::occluded(S, P.use_bvh != 0, o, d, B.stm[k], lds_stack<STACK>(sref, stn), c);
     if (!occ) {
-      B.lx[pid] = B.lx[pid] + B.scx[i];
-      B.ly[pid] = B.ly[pid] + B.scy[i];
-      B.lz[pid] = B.lz[pid] + B.scz[i];
+      B.lx[pid] = B.lx[pid] + B.scx[k];
+      B.ly[pid] = B.ly[pid] + B.scy[k];
+      B.lz[pid] = B.lz[pid] + B.scz[k];
     }
     st[kStSVisits] = c.visits;
     st[kStSTests] = c.tests;
@@ -337,7 +428,7 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(PathBuffers<T> B, ChunkPa
     double sx = P.acc_sum[3 * (size_t)pixel], sy = P.acc_sum[3 * (size_t)pixel + 1],
            sz = P.acc_sum[3 * (size_t)pixel + 2];
     for (uint32_t s = 0; s < P.k; ++s) {
-      const uint32_t pid = j * P.k + s;
+      const uint32_t pid = pid_of(P, j, s);
       T x = B.lx[pid], y = B.ly[pid], z = B.lz[pid];
       if (!(r_isfinite(x) && r_isfinite(y) && r_isfinite(z)) || x < 0 || y < 0 || z < 0) {
         ++st[kStInvalid];
@@ -498,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_direct(DevScene S, PathBuffers<T> B,
   const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t st[kStCount] = {0};
   if (pid < P.n_paths) {
-    const uint32_t pixel = P.p0 + pid / P.k, sample = P.s0 + pid % P.k;
+    const uint32_t pixel = pid_pixel(P, pid), sample = pid_sample(P, pid);
     const int x = (int)(pixel % P.width), y = (int)(pixel / P.width);
     V3<T> o, d, L;
     if constexpr (sizeof(T) == 8) {
