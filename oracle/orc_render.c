@@ -410,3 +410,29 @@ void orc_resolve(const double* sum, const uint32_t* count, int w, int h, int op,
     rgb[k++] = orc_linear_to_srgb8(out[2]);
   }
 }
+
+/* The torus ray set: n rays from Rng(seed) (math/rng.hpp:22-27); origin =
+ * c + e * U(-1,1)^3 with c, e the centroid and extent of the BVH root box
+ * (a box twice the scene's), direction = random_unit_vector
+ * (sampling.hpp:30-35); every component rounded to float, then widened
+ * back to double, so FP32 and FP64 traversals see the same rays. */
+void orc_ray_set(uint64_t seed, uint64_t n, const double lo[3], const double hi[3], double* rays) {
+  orc_rng rng;
+  orc_rng_init(&rng, seed, 0xda3e39cb94b95bdbull);
+  const v3 c = vscale(vadd(vload(lo), vload(hi)), 0.5), e = vsub(vload(hi), vload(lo));
+  for (uint64_t i = 0; i < n; ++i) {
+    double* r = rays + 8 * i;
+    const double ux = -1 + 2 * orc_rng_uniform(&rng);
+    const double uy = -1 + 2 * orc_rng_uniform(&rng);
+    const double uz = -1 + 2 * orc_rng_uniform(&rng);
+    const v3 d = random_unit_vector(&rng);
+    r[0] = (double)(float)(c.x + e.x * ux);
+    r[1] = (double)(float)(c.y + e.y * uy);
+    r[2] = (double)(float)(c.z + e.z * uz);
+    r[3] = (double)(float)d.x;
+    r[4] = (double)(float)d.y;
+    r[5] = (double)(float)d.z;
+    r[6] = 1e-6;
+    r[7] = INFINITY;
+  }
+}

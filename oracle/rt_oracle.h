@@ -93,6 +93,9 @@ void orc_render_rows(const rtg_scene_desc* s, const rtg_render_settings* cfg, ui
 void orc_sample_radiance_rows(const rtg_scene_desc* s, const rtg_render_settings* cfg, uint32_t s0, uint32_t s1,
                               int y0, int y1, double* out, rtg_stats* st);
 
+/* Seeded ray set over twice the BVH root box (see orc_render.c). */
+void orc_ray_set(uint64_t seed, uint64_t n, const double lo[3], const double hi[3], double* rays);
+
 /* ---- Tone map: src/rendering/tonemap.cpp ---- */
 void orc_tone_map(const double in[3], int op, double exposure, double out[3]);
 uint8_t orc_linear_to_srgb8(double v);

@@ -110,6 +110,33 @@ def main():
     out["hits_torus_closest"] = h.view(np.uint8).reshape(len(h), -1)
     meta["hits_torus"] = {"node_visits": nv, "prim_tests": pt}
 
+    # 3b. The 2^24-ray torus set (orc_ray_set, seed 7) through Scene::intersect:
+    # digests of every hit id and t, hit count and traversal counters.
+    torus = orc.RefScene(json.dumps(scenes.torus_only()))
+    rays = orc.ray_set(torus.desc, 1 << 24, seed=7)
+    hits = np.zeros(len(rays), orc.rtg.HIT_DTYPE)
+    parts = np.array_split(np.arange(len(rays)), 8)
+    counters = [(C.c_uint64(0), C.c_uint64(0)) for _ in parts]
+
+    def run(k):
+        idx = parts[k]
+        sub = np.ascontiguousarray(rays[idx])
+        out = np.zeros(len(sub), orc.rtg.HIT_DTYPE)
+        L.ref_trace_batch(torus.h, 0, 1, sub.ctypes.data_as(C.c_void_p), len(sub), out.ctypes.data_as(C.c_void_p),
+                          C.byref(counters[k][0]), C.byref(counters[k][1]))
+        hits[idx] = out
+
+    import threading
+    ts = [threading.Thread(target=run, args=(k,)) for k in range(len(parts))]
+    [t.start() for t in ts]
+    [t.join() for t in ts]
+    id_sha, t_sha = orc.ids_digest(hits)
+    meta["ray_set_2p24"] = {"ids_sha256": id_sha, "t_sha256": t_sha, "hits": int((hits["flags"] & 1).sum()),
+                            "node_visits": int(sum(c[0].value for c in counters)),
+                            "prim_tests": int(sum(c[1].value for c in counters)),
+                            "first_rays_sha256": hashlib.sha256(rays[:4096].tobytes()).hexdigest()}
+    out["ray_set_head_hits"] = hits[:65536].view(np.uint8).reshape(65536, -1)
+
     # 4. Framebuffer sums (render_tile's sample loop through radiance()).
     for name, w, h_, spp, ov in [("cornell_c1.json", 16, 16, 4, {}), ("torus_c2.json", 12, 12, 2, {})]:
         r = orc.RefScene(json.dumps(scenes.load(name)), width=w, height=h_, **ov)

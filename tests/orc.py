@@ -99,6 +99,7 @@ def orc():
                                              P(P(C.c_uint32)), P(C.c_uint32)]),
             "orc_free": (None, [VP]),
             "orc_random_scene_objects": (C.c_uint32, [C.c_int, C.c_int, C.c_uint64, C.c_int, P(OrcObject)]),
+            "orc_ray_set": (None, [C.c_uint64, C.c_uint64, P(C.c_double), P(C.c_double), VP]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -369,6 +370,23 @@ class RefScene:
         [t.start() for t in ts]
         [t.join() for t in ts]
         return s, c, [sum(st[i] for st in stats) for i in range(5)]
+
+
+def ray_set(desc, n, seed=7):
+    """The seeded ray set over twice the BVH root box (orc_ray_set)."""
+    root = desc.nodes[0]
+    lo = (C.c_double * 3)(root.lo.x, root.lo.y, root.lo.z)
+    hi = (C.c_double * 3)(root.hi.x, root.hi.y, root.hi.z)
+    rays = np.zeros((n, 8))
+    orc().orc_ray_set(seed, n, lo, hi, C.c_void_p(rays.ctypes.data))
+    return rays
+
+
+def ids_digest(hits):
+    """SHA-256 of the hit primitive ids and of the t bits of a hit array."""
+    import hashlib
+    return (hashlib.sha256(np.ascontiguousarray(hits["prim"]).tobytes()).hexdigest(),
+            hashlib.sha256(np.ascontiguousarray(hits["t"]).view(np.uint64).tobytes()).hexdigest())
 
 
 # ---------------------------------------------------------------------------

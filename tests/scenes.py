@@ -19,6 +19,15 @@ def load_asset(name):
         return json.load(f)
 
 
+def torus_only():
+    """The 480x180 torus of the torus workload alone (no plane, no lights):
+    the scene of the 2^24-ray hit-id check."""
+    j = load("torus_c2.json")
+    j["objects"] = [o for o in j["objects"] if o["type"] == "mesh"]
+    j["lights"] = []
+    return j
+
+
 def sphere_light_over_plane(nee=True):
     """tests/test_render.cpp:27-45 — a*Le*(r/d)^2 = 0.2 below a sphere light."""
     return {

@@ -1,0 +1,64 @@
+"""The 2^24-ray hit-id check on the 172,800-triangle torus: seeded rays over
+twice the scene box (tests/orc.py ray_set), closest hit on the GPU.
+
+FP64 mode must reproduce the reference's Scene::intersect for every ray —
+the SHA-256 of all 16,777,216 hit ids and of all t values, the hit count and
+the traversal counters were computed by the reference build
+(tests/golden/make_golden.py). Brute force (--no-bvh) must give the same ids
+as the BVH. FP32 mismatches are counted and bounded."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import orc
+import scenes
+from _pkg import rtg
+
+pytestmark = pytest.mark.gpu
+META = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_meta.json")))["ray_set_2p24"]
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz"))
+
+
+@pytest.fixture(scope="module")
+def torus(ctx):
+    hs = rtg.HostScene(json.dumps(scenes.torus_only()))
+    assert hs.info()["triangles"] == 172800
+    rays = orc.ray_set(hs.desc, 1 << 24, seed=7)
+    import hashlib
+    assert hashlib.sha256(rays[:4096].tobytes()).hexdigest() == META["first_rays_sha256"]
+    dev = ctx.scene(hs.desc)
+    yield hs, dev, rays
+    dev.release()
+
+
+def test_f64_all_rays_match_reference(ctx, torus):
+    hs, dev, rays = torus
+    hits, st = ctx.trace(dev, rays, mode=0, use_bvh=1, precision=rtg.PRECISION_F64)
+    ids, ts = orc.ids_digest(hits)
+    print(f"2^24 rays fp64: {st.elapsed_ms:.1f} ms device, {st.node_visits / len(rays):.2f} nodes/ray", flush=True)
+    assert int((hits["flags"] & 1).sum()) == META["hits"]
+    assert (st.node_visits, st.prim_tests) == (META["node_visits"], META["prim_tests"])
+    assert ids == META["ids_sha256"] and ts == META["t_sha256"]
+
+
+def test_brute_force_matches_bvh(ctx, torus):
+    hs, dev, rays = torus
+    head = rays[:65536]
+    hb, st = ctx.trace(dev, head, mode=0, use_bvh=0, precision=rtg.PRECISION_F64)
+    print(f"brute force fp64: {len(head)} rays x 172800 in {st.elapsed_ms:.1f} ms", flush=True)
+    want = GOLD["ray_set_head_hits"].view(rtg.HIT_DTYPE).reshape(-1)
+    assert np.array_equal(hb["prim"], want["prim"])
+    assert np.array_equal(hb["t"].view(np.uint64), want["t"].view(np.uint64))
+
+
+def test_f32_mismatches_are_rare(ctx, torus):
+    hs, dev, rays = torus
+    h64, _ = ctx.trace(dev, rays, mode=0, precision=rtg.PRECISION_F64)
+    h32, st = ctx.trace(dev, rays, mode=0, precision=rtg.PRECISION_F32)
+    mism = int((h32["prim"] != h64["prim"]).sum())
+    print(f"2^24 rays fp32: {st.elapsed_ms:.1f} ms device, id mismatches vs fp64: {mism}", flush=True)
+    assert mism <= len(rays) * 1e-4
+    both = (h32["prim"] == h64["prim"]) & ((h64["flags"] & 1) != 0)
+    np.testing.assert_allclose(h32["t"][both], h64["t"][both], rtol=1e-5, atol=1e-6)

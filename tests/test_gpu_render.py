@@ -53,6 +53,18 @@ def test_f64_render_matches_oracle(ctx, name, w, h, spp):
     assert abs(stg[0]["shadow_rays"] - sto["shadow_rays"]) <= max(2, 0.002 * sto["shadow_rays"] + 2)
 
 
+@pytest.mark.parametrize("name,w,h,spp", [("cornell_c1.json", 24, 24, 4), ("torus_c2.json", 24, 24, 2),
+                                          ("sphere_field_c3.json", 32, 18, 2)])
+def test_f64_workload_scenes_match_oracle(ctx, name, w, h, spp):
+    o = orc.OracleScene(scenes.load(name), width=w, height=h)
+    so, co, sto = o.render(0, spp)
+    sg, cg, stg = gpu_render(ctx, o, 0, spp, rtg.PRECISION_F64)
+    assert np.array_equal(cg, co)
+    frac = rel_close_fraction(sg, so, 1e-9)
+    print(name, "fraction within 1e-9:", frac, flush=True)
+    assert frac >= 0.99
+
+
 def test_f64_render_passes_accumulate_in_sample_order(ctx):
     o = orc.OracleScene(scenes.load_asset("cornell_box.json"), width=16, height=16)
     s1, c1, _ = gpu_render(ctx, o, 0, 8, rtg.PRECISION_F64)
