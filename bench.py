@@ -95,11 +95,25 @@ def main():
     ap.add_argument("--workload", default="torus_c2", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (3 times): the reference kernel "
+                         "that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build renders on one GPU; multi-GPU runs are not part of it "
               "(the sample range would shard across GPUs with one FP32 sum+count reduction)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe > 0:
+        import torch
+        n = args.copy_probe << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "copies": 3}))
+        return
     rtg = load_pkg()
     with open(os.path.join(PKG, "scenes", WORKLOADS[args.workload])) as f:
         text = f.read()

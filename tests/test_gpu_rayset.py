@@ -61,9 +61,20 @@ def test_f32_mismatches_are_rare(ctx, torus):
     print(f"2^24 rays fp32: {st.elapsed_ms:.1f} ms device, id mismatches vs fp64: {mism}", flush=True)
     assert mism <= len(rays) * 1e-4
     both = (h32["prim"] == h64["prim"]) & ((h64["flags"] & 1) != 0)
-    rel = np.abs(h32["t"][both] - h64["t"][both]) / np.abs(h64["t"][both])
-    within = float((rel <= 1e-5).mean())
-    print(f"fp32 t within 1e-5 relative: {within:.6f}, max rel {rel.max():.2e}", flush=True)
-    # Grazing hits (ray nearly in the triangle's plane) are ill-conditioned in
-    # float; the bound is on the fraction, the FP64 mode is bit-exact.
-    assert within >= 0.9999
+    # FP32 hit points carry float rounding at the scene's scale (extent ~1.35,
+    # ulp ~1.2e-7): the error of t projected on the triangle's normal, i.e. the
+    # hit point's distance from the true surface, is bounded by a few tens of
+    # ulps. Relative error of t alone is not: origins lie anywhere in twice the
+    # scene box, so t can be ~1e-3 with the same absolute error.
+    prims = orc.arrays(hs.desc)["prims"].view(np.float64).reshape(-1, 20)
+    idx = h64["prim"][both].astype(np.int64)
+    v0, v1, v2 = prims[idx, 2:5], prims[idx, 5:8], prims[idx, 8:11]
+    n = np.cross(v1 - v0, v2 - v0)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    cos = np.abs((n * rays[both, 3:6]).sum(1))
+    dt = np.abs(h32["t"][both] - h64["t"][both])
+    off = dt * cos
+    print(f"fp32 |dt|: max {dt.max():.2e}; distance from surface: max {off.max():.2e}; "
+          f"{(dt <= 1e-5).mean():.6f} of hits within 1e-5", flush=True)
+    assert off.max() <= 4e-6
+    assert (dt <= 1e-5).mean() >= 0.9999
