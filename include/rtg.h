@@ -202,6 +202,26 @@ rtg_status rtg_render_pass(rtg_context* ctx, const rtg_scene* scene, const rtg_r
  * writes width*height*3 bytes. tone_map: 0 clamp, 1 reinhard, 2 aces. */
 rtg_status rtg_resolve(rtg_context* ctx, const rtg_accum* acc, int32_t tone_map, double exposure, uint8_t* rgb);
 
+/* Batched rendering: many jobs — each one pass (samples [s0, s1) of pixels
+ * [p0, p1)) of one render with its own scene, settings and accumulation
+ * buffer — rendered together. FP32 path-integrator jobs with max_depth <= 64
+ * share the path arrays and every kernel launch (planned on the device, so a
+ * batch of small renders fills the GPU like one large one); other jobs run
+ * through the single-render path in the same call. Every job's result is
+ * bitwise the result of rendering it alone with rtg_render_pass.
+ * Jobs that write the same accumulation buffer must cover disjoint pixel
+ * ranges. stats (n_jobs entries, may be NULL) receives each job's counters;
+ * elapsed_ms is the device time of the whole call. The reference renders one
+ * scene per Renderer (renderer.cpp:58-105); this has no counterpart there. */
+typedef struct rtg_job {
+  const rtg_scene* scene;
+  rtg_accum* acc;
+  rtg_render_settings settings; /* width/height = the accumulation buffer's */
+  uint32_t s0, s1;              /* samples [s0, s1) */
+  uint32_t p0, p1;              /* pixels [p0, p1), row-major; p1 = 0: all */
+} rtg_job;
+rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jobs, rtg_stats* stats);
+
 /* ------------------------------------------------------------------------ */
 /* Batch ray queries: Scene::intersect / occluded / intersect_brute
  * (src/core/scene.cpp:176-234) over n rays.
@@ -241,9 +261,10 @@ rtg_status rtg_synchronize(rtg_context* ctx);
 /* Per-kernel device timing of the render passes and traces issued since
  * profiling was switched on (HIP events around every launch, on the
  * context's stream). kind: 0 generate, 1 extend (closest hit), 2 shade,
- * 3 shadow (any hit), 4 accumulate, 5 direct/normals sample, 6 trace. */
+ * 3 shadow (any hit), 4 accumulate, 5 direct/normals sample, 6 trace,
+ * 7 batch planning. */
 enum { RTG_K_GENERATE = 0, RTG_K_EXTEND, RTG_K_SHADE, RTG_K_SHADOW, RTG_K_ACCUMULATE, RTG_K_DIRECT, RTG_K_TRACE,
-       RTG_K_COUNT };
+       RTG_K_PLAN, RTG_K_COUNT };
 typedef struct rtg_profile {
   uint64_t launches[RTG_K_COUNT];
   double ms[RTG_K_COUNT];

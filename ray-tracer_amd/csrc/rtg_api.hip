@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -17,6 +18,13 @@ using namespace rtg;
 namespace {
 thread_local std::string g_error;
 
+// Path-state slots of a context.
+constexpr int kBatchBuffers = 2;
+// Batches: every job's queue storage is rounded up to kSubQueues sub-queues
+// of whole workgroups, at most kSubQueues * kBlock entries more than its
+// paths.
+constexpr uint64_t kBatchQueueSlack = (uint64_t)kMaxBatchJobs * kSubQueues * kBlock;
+
 struct Status {
   rtg_status code;
   std::string msg;
@@ -29,12 +37,19 @@ struct rtg_context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
   unsigned long long* stats = nullptr;
-  // Path state, grown on demand per precision.
-  void* path_mem[2] = {nullptr, nullptr};
-  uint64_t path_bytes[2] = {0, 0};
+  int n_cu = 0;
+  // Path state, grown on demand: FP32 chunks, FP64 chunks, FP32 batches.
+  void* path_mem[3] = {nullptr, nullptr, nullptr};
+  uint64_t path_bytes[3] = {0, 0, 0};
   PathBuffers<float> pb32{};
   PathBuffers<double> pb64{};
-  uint32_t cap[2] = {0, 0};
+  PathBuffers<float> pbb{};
+  uint32_t cap[3] = {0, 0, 0};
+  // Batches: device job table and plans, pinned host staging of both.
+  BatchJob* batch_jobs = nullptr;
+  uint32_t* batch_plans = nullptr;  // 4 x (kMaxBatchJobs + 1): gen, acc, act, shd
+  void* batch_host = nullptr;       // job table + gen/acc plans, pinned
+  uint32_t batch_occ[3][3] = {};    // per stack class: trace, shade, other workgroups per CU
   // Profiling: an event pair per launch, resolved when the call completes.
   bool profiling = false;
   std::vector<hipEvent_t> events;
@@ -225,6 +240,7 @@ rtg_status rtg_open(int device, rtg_context** out) {
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e == hipSuccess) e = hipMalloc(&c->stats, sizeof(unsigned long long) * kStatSlots * kStatStride);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) {
     const std::string m = std::string("rtg_open: ") + hipGetErrorString(e);
     delete c;
@@ -241,6 +257,9 @@ rtg_status rtg_close(rtg_context* ctx) {
   for (void* p : ctx->path_mem)
     if (p) hipFree(p);
   if (ctx->stats) hipFree(ctx->stats);
+  if (ctx->batch_jobs) hipFree(ctx->batch_jobs);
+  if (ctx->batch_plans) hipFree(ctx->batch_plans);
+  if (ctx->batch_host) hipHostFree(ctx->batch_host);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
   for (hipEvent_t e : ctx->events) hipEventDestroy(e);
@@ -560,6 +579,8 @@ rtg_status rtg_resolve(rtg_context* ctx, const rtg_accum* a, int32_t tone_map, d
 // ---------------------------------------------------------------------------
 namespace {
 
+// Path state for `cap` paths. Every array has c entries: c = room for all
+// sub-queues of one chunk, or (batches) of every job of a batch.
 template <class T>
 rtg_status ensure_paths(rtg_context* ctx, int which, uint32_t cap, PathBuffers<T>& B) {
   if (ctx->cap[which] >= cap) return RTG_OK;
@@ -570,12 +591,17 @@ rtg_status ensure_paths(rtg_context* ctx, int which, uint32_t cap, PathBuffers<T
     ctx->cap[which] = 0;
   }
   const uint32_t sub_cap = sub_queue_cap(cap);
-  const uint64_t c = (uint64_t)sub_cap * kSubQueues;  // >= cap: room for every sub-queue
+  const uint64_t c = which == kBatchBuffers ? (uint64_t)cap + kBatchQueueSlack
+                                            : (uint64_t)sub_cap * kSubQueues;  // >= cap
   auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
   const uint64_t tbytes = al(c * sizeof(T));
   // 6 ray + 3 beta + 3 L + 3 hit + 10 shadow arrays of T; 2 RNG u64; 5 u32
   // (2 queues, hprim, hplane, sq_pid); spec; counts.
-  const uint64_t bytes = tbytes * 25 + al(c * 8) * 2 + al(c * 4) * 5 + al(c) + al(sizeof(uint32_t) * 2 * kMaxBounce * kSubQueues);
+  const bool batch = which == kBatchBuffers;
+  const uint64_t n_counts = batch ? (uint64_t)kMaxBatchJobs * kBatchCounts : 2ull * kMaxBounce * kSubQueues;
+  const uint64_t n_stats = batch ? (uint64_t)kMaxBatchJobs * kBatchStatSlots * kStatStride : 0;
+  const uint64_t bytes = tbytes * 25 + al(c * 8) * 2 + al(c * 4) * 5 + al(c) + al(sizeof(uint32_t) * n_counts) +
+                         al(8 * n_stats);
   void* mem = nullptr;
   HIP_TRY(ctx, hipMalloc(&mem, bytes));
   char* p = static_cast<char*>(mem);
@@ -595,8 +621,8 @@ rtg_status ensure_paths(rtg_context* ctx, int which, uint32_t cap, PathBuffers<T
   B.hplane = reinterpret_cast<int32_t*>(take(c * 4));
   B.sq_pid = reinterpret_cast<uint32_t*>(take(c * 4));
   B.spec = reinterpret_cast<uint8_t*>(take(c));
-  B.counts = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 2 * kMaxBounce * kSubQueues));
-  B.stats = ctx->stats;
+  B.counts = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * n_counts));
+  B.stats = batch ? reinterpret_cast<unsigned long long*>(take(8 * n_stats)) : ctx->stats;
   B.capacity = cap;
   B.sub_cap = sub_cap;
   ctx->path_mem[which] = mem;
@@ -621,6 +647,16 @@ int32_t pid_layout_default() {
   return 0;
 }
 
+void add_counters(rtg_stats* out, const unsigned long long* h) {
+  out->camera_samples += h[kStCamera];
+  out->rays += h[kStRays];
+  out->shadow_rays += h[kStShadow];
+  out->rr_terminated += h[kStRR];
+  out->invalid_samples += h[kStInvalid];
+  out->node_visits += h[kStVisits] + h[kStSVisits];
+  out->prim_tests += h[kStTests] + h[kStSTests];
+}
+
 void read_stats(rtg_context* ctx, rtg_stats* out, float ms) {
   static thread_local std::vector<unsigned long long> slots;
   slots.assign((size_t)kStatSlots * kStatStride, 0);
@@ -630,40 +666,57 @@ void read_stats(rtg_context* ctx, rtg_stats* out, float ms) {
     for (int k = 0; k < kStCount; ++k) h[k] += slots[(size_t)sl * kStatStride + k];
   profile_resolve(ctx, h);
   if (!out) return;
-  out->camera_samples = h[kStCamera];
-  out->rays = h[kStRays];
-  out->shadow_rays = h[kStShadow];
-  out->rr_terminated = h[kStRR];
-  out->invalid_samples = h[kStInvalid];
-  out->node_visits = h[kStVisits] + h[kStSVisits];
-  out->prim_tests = h[kStTests] + h[kStSTests];
+  *out = rtg_stats{};
+  add_counters(out, h);
   out->elapsed_ms = ms;
 }
 
-}  // namespace
-
-extern "C" {
-
-rtg_status rtg_render_pass(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* cfg, uint32_t s0,
-                           uint32_t s1, rtg_accum* acc, const volatile int32_t* cancel, rtg_stats* stats) {
-  if (!ctx || !scene || !cfg || !acc) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_pass: NULL argument");
+rtg_status check_settings(rtg_context* ctx, const std::string& fn, const rtg_render_settings* cfg,
+                          const rtg_accum* acc, uint32_t s0, uint32_t s1) {
   if (cfg->width != acc->w || cfg->height != acc->h)
-    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_pass: settings size differs from the accumulation buffer");
-  if (s1 <= s0) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_pass: empty sample range");
+    return fail(ctx, RTG_ERR_INVALID_ARG, fn + ": settings size differs from the accumulation buffer");
+  if (s1 <= s0) return fail(ctx, RTG_ERR_INVALID_ARG, fn + ": empty sample range");
   if (cfg->max_depth < 1 || cfg->max_depth >= kMaxBounce)
-    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_pass: max_depth must be in [1, 1023]");
+    return fail(ctx, RTG_ERR_INVALID_ARG, fn + ": max_depth must be in [1, 1023]");
   if (cfg->integrator < 0 || cfg->integrator > 2 || cfg->precision < 0 || cfg->precision > 1)
-    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_pass: bad integrator or precision");
+    return fail(ctx, RTG_ERR_INVALID_ARG, fn + ": bad integrator or precision");
   if (cfg->integrator == RTG_INTEGRATOR_DIRECT && cfg->max_depth > 31)
-    return fail(ctx, RTG_ERR_UNSUPPORTED, "rtg_render_pass: the direct integrator supports max_depth <= 31");
-  HIP_TRY(ctx, hipSetDevice(ctx->device));
-  const bool f64 = cfg->precision == RTG_PRECISION_F64;
-  // Chunk capacity: the configured maximum, or less when the whole pass is
-  // smaller (rounded up to a power of two so buffers are reused across passes).
-  const uint64_t pass_paths = (uint64_t)(s1 - s0) * (uint64_t)acc->w * (uint64_t)acc->h;
+    return fail(ctx, RTG_ERR_UNSUPPORTED, fn + ": the direct integrator supports max_depth <= 31");
+  return RTG_OK;
+}
+
+// The per-render fields of a job's parameters.
+ChunkParams render_params(const rtg_render_settings* cfg, const rtg_accum* acc) {
+  ChunkParams P{};
+  P.width = (uint32_t)acc->w;
+  P.height = (uint32_t)acc->h;
+  P.max_depth = cfg->max_depth;
+  P.rr_min_depth = cfg->rr_min_depth;
+  P.integrator = cfg->integrator;
+  P.light_sampling = cfg->light_sampling;
+  P.use_bvh = cfg->use_bvh;
+  P.sample_major = pid_layout_default();
+  P.seed = cfg->seed;
+  P.acc_sum = acc->sum;
+  P.acc_count = acc->count;
+  return P;
+}
+
+// Chunk capacity: the configured maximum, or less when the work is smaller
+// (rounded up to a power of two so buffers are reused across passes).
+uint32_t capacity_for(uint64_t paths) {
+  const uint32_t lim = chunk_paths_default();
   uint32_t cap = 1024;
-  while (cap < pass_paths && cap < chunk_paths_default()) cap <<= 1;
-  cap = std::min(cap, chunk_paths_default());
+  while (cap < paths && cap < lim) cap <<= 1;
+  return std::min(cap, lim);
+}
+
+// Samples [s0, s1) of pixels [px0, px1), one chunk of whole pixels at a time.
+rtg_status render_range(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* cfg, uint32_t s0,
+                        uint32_t s1, uint32_t px0, uint32_t px1, rtg_accum* acc, const volatile int32_t* cancel,
+                        rtg_stats* stats, const char* fn) {
+  const bool f64 = cfg->precision == RTG_PRECISION_F64;
+  const uint32_t cap = capacity_for((uint64_t)(s1 - s0) * (px1 - px0));
   if (f64) {
     if (rtg_status st = ensure_paths(ctx, 1, cap, ctx->pb64)) return st;
   } else {
@@ -672,36 +725,30 @@ rtg_status rtg_render_pass(rtg_context* ctx, const rtg_scene* scene, const rtg_r
   HIP_TRY(ctx, hipMemsetAsync(ctx->stats, 0, sizeof(unsigned long long) * kStatSlots * kStatStride, ctx->stream));
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   const uint32_t k_total = s1 - s0;
-  const uint32_t n_pix = (uint32_t)acc->w * (uint32_t)acc->h;
   bool cancelled = false;
   // Sample sub-ranges outermost keep every pixel's additions in sample order.
   for (uint32_t sa = s0; sa < s1 && !cancelled;) {
     const uint32_t k = std::min(k_total - (sa - s0), cap);
     const uint32_t pix_per_chunk = std::max(1u, cap / k);
-    for (uint32_t p0 = 0; p0 < n_pix; p0 += pix_per_chunk) {
+    for (uint32_t p0 = px0; p0 < px1; p0 += pix_per_chunk) {
       if (cancel && *cancel) {
         cancelled = true;
         break;
       }
-      ChunkParams P{};
-      P.width = (uint32_t)acc->w;
-      P.height = (uint32_t)acc->h;
+      ChunkParams P = render_params(cfg, acc);
       P.p0 = p0;
       P.k = k;
       P.s0 = sa;
-      P.n_pix = std::min(pix_per_chunk, n_pix - p0);
+      P.n_pix = std::min(pix_per_chunk, px1 - p0);
       P.n_paths = P.n_pix * k;
-      P.max_depth = cfg->max_depth;
-      P.rr_min_depth = cfg->rr_min_depth;
-      P.integrator = cfg->integrator;
-      P.light_sampling = cfg->light_sampling;
-      P.use_bvh = cfg->use_bvh;
-      P.sample_major = pid_layout_default();
-      P.seed = cfg->seed;
-      P.acc_sum = acc->sum;
-      P.acc_count = acc->count;
+      const PathBuffers<float>& B32 = ctx->pb32;
+      P.sub_cap = f64 ? ctx->pb64.sub_cap : B32.sub_cap;
+      P.shadow_counts = kMaxBounce * kSubQueues;
+      P.stat_slots = kStatSlots;
+      P.counts = f64 ? ctx->pb64.counts : B32.counts;
+      P.stats = ctx->stats;
       if (f64) run_chunk(scene->dev, ctx->pb64, P, scene->stack, ctx->stream, hooks_of(ctx));
-      else run_chunk(scene->dev, ctx->pb32, P, scene->stack, ctx->stream, hooks_of(ctx));
+      else run_chunk(scene->dev, B32, P, scene->stack, ctx->stream, hooks_of(ctx));
       HIP_TRY(ctx, hipGetLastError());
       if (cancel) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // poll between chunks
     }
@@ -712,7 +759,191 @@ rtg_status rtg_render_pass(rtg_context* ctx, const rtg_scene* scene, const rtg_r
   float ms = 0;
   hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
   read_stats(ctx, stats, ms);
-  if (cancelled) return fail(ctx, RTG_ERR_CANCELLED, "rtg_render_pass: cancelled");
+  if (cancelled) return fail(ctx, RTG_ERR_CANCELLED, std::string(fn) + ": cancelled");
+  return RTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches
+struct Segment {
+  uint32_t job, p0, n_pix, s0, k;
+};
+
+bool batchable(const rtg_job& j) {
+  return j.settings.precision == RTG_PRECISION_F32 && j.settings.integrator == RTG_INTEGRATOR_PATH &&
+         j.settings.max_depth <= kBatchDepth;
+}
+
+rtg_status ensure_batch_tables(rtg_context* ctx) {
+  if (ctx->batch_jobs) return RTG_OK;
+  HIP_TRY(ctx, hipMalloc(&ctx->batch_jobs, sizeof(BatchJob) * kMaxBatchJobs));
+  HIP_TRY(ctx, hipMalloc(&ctx->batch_plans, sizeof(uint32_t) * 4 * (kMaxBatchJobs + 1)));
+  HIP_TRY(ctx, hipHostMalloc(&ctx->batch_host, sizeof(BatchJob) * kMaxBatchJobs + sizeof(uint32_t) * 2 * (kMaxBatchJobs + 1)));
+  return RTG_OK;
+}
+
+int stack_class(uint32_t stack) { return stack <= kStackClasses[0] ? 0 : stack <= kStackClasses[1] ? 1 : 2; }
+
+// One batched pass over segments seg[0..n) (n <= kMaxBatchJobs, total paths
+// <= the batch capacity). Adds each segment's counters to its job's stats.
+rtg_status run_group(rtg_context* ctx, const rtg_job* jobs, const Segment* seg, uint32_t n,
+                     std::vector<rtg_stats>& acc_stats) {
+  auto* hj = static_cast<BatchJob*>(ctx->batch_host);
+  auto* hgen = reinterpret_cast<uint32_t*>(hj + kMaxBatchJobs);
+  uint32_t* hacc = hgen + (kMaxBatchJobs + 1);
+  const PathBuffers<float>& B = ctx->pbb;
+  uint64_t pid = 0, q = 0;
+  uint32_t gen_blocks = 0, acc_blocks = 0, path_blocks = 0, stack = 1;
+  int depth = 1;
+  for (uint32_t i = 0; i < n; ++i) {
+    const rtg_job& J = jobs[seg[i].job];
+    ChunkParams P = render_params(&J.settings, J.acc);
+    P.p0 = seg[i].p0;
+    P.k = seg[i].k;
+    P.s0 = seg[i].s0;
+    P.n_pix = seg[i].n_pix;
+    P.n_paths = P.n_pix * P.k;
+    P.pid_base = (uint32_t)pid;
+    P.q_base = (uint32_t)q;
+    P.sub_cap = sub_queue_cap(P.n_paths);
+    P.shadow_counts = kBatchShadowCounts;
+    P.stat_slots = kBatchStatSlots;
+    P.zero_stats = 1;
+    P.counts = B.counts + (size_t)i * kBatchCounts;
+    P.stats = B.stats + (size_t)i * kBatchStatSlots * kStatStride;
+    pid += P.n_paths;
+    q += (uint64_t)P.sub_cap * kSubQueues;
+    hj[i].S = J.scene->dev;
+    hj[i].P = P;
+    hgen[i] = gen_blocks;
+    gen_blocks += (generate_threads(P) + kBlock - 1) / kBlock;
+    hacc[i] = acc_blocks;
+    acc_blocks += (P.n_pix + kBlock - 1) / kBlock;
+    path_blocks += (P.n_paths + kBlock - 1) / kBlock;
+    depth = std::max(depth, P.max_depth);
+    stack = std::max(stack, J.scene->stack);
+  }
+  hgen[n] = gen_blocks;
+  hacc[n] = acc_blocks;
+  if (pid > ctx->cap[kBatchBuffers] || q > (uint64_t)ctx->cap[kBatchBuffers] + kBatchQueueSlack)
+    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_batch: internal error: group exceeds the batch buffers");
+  uint32_t* plans = ctx->batch_plans;
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->batch_jobs, hj, sizeof(BatchJob) * n, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(plans, hgen, sizeof(uint32_t) * 2 * (kMaxBatchJobs + 1), hipMemcpyHostToDevice,
+                              ctx->stream));
+  BatchPlans L{plans, plans + (kMaxBatchJobs + 1), plans + 2 * (kMaxBatchJobs + 1), plans + 3 * (kMaxBatchJobs + 1), n};
+  uint32_t* occ = ctx->batch_occ[stack_class(stack)];
+  if (!occ[0]) batch_occupancy(stack, &occ[0], &occ[1], &occ[2]);
+  const uint32_t cu = (uint32_t)std::max(1, ctx->n_cu);
+  BatchGrids g;
+  g.generate = std::max(1u, std::min(gen_blocks, occ[2] * cu));
+  g.trace = std::max(1u, std::min(path_blocks, occ[0] * cu));
+  g.shade = std::max(1u, std::min(path_blocks, occ[1] * cu));
+  g.accumulate = std::max(1u, std::min(acc_blocks, occ[2] * cu));
+  run_batch(ctx->batch_jobs, B, L, depth, stack, g, ctx->stream, hooks_of(ctx));
+  HIP_TRY(ctx, hipGetLastError());
+  static thread_local std::vector<unsigned long long> slots;
+  slots.assign((size_t)n * kBatchStatSlots * kStatStride, 0);
+  HIP_TRY(ctx, hipMemcpyAsync(slots.data(), B.stats, slots.size() * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  unsigned long long tot[kStCount] = {0};
+  for (uint32_t i = 0; i < n; ++i) {
+    unsigned long long h[kStCount] = {0};
+    for (uint32_t sl = 0; sl < kBatchStatSlots; ++sl)
+      for (int k = 0; k < kStCount; ++k) h[k] += slots[((size_t)i * kBatchStatSlots + sl) * kStatStride + k];
+    add_counters(&acc_stats[seg[i].job], h);
+    for (int k = 0; k < kStCount; ++k) tot[k] += h[k];
+  }
+  profile_resolve(ctx, tot);
+  return RTG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rtg_status rtg_render_pass(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* cfg, uint32_t s0,
+                           uint32_t s1, rtg_accum* acc, const volatile int32_t* cancel, rtg_stats* stats) {
+  if (!ctx || !scene || !cfg || !acc) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_pass: NULL argument");
+  if (rtg_status st = check_settings(ctx, "rtg_render_pass", cfg, acc, s0, s1)) return st;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  return render_range(ctx, scene, cfg, s0, s1, 0, (uint32_t)acc->w * (uint32_t)acc->h, acc, cancel, stats,
+                      "rtg_render_pass");
+}
+
+rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jobs, rtg_stats* stats) {
+  if (!ctx || (n_jobs && !jobs)) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_batch: NULL argument");
+  const std::string fn = "rtg_render_batch";
+  std::vector<uint32_t> p0(n_jobs), p1(n_jobs);
+  for (uint32_t i = 0; i < n_jobs; ++i) {
+    const rtg_job& J = jobs[i];
+    const std::string fi = fn + ": job " + std::to_string(i);
+    if (!J.scene || !J.acc) return fail(ctx, RTG_ERR_INVALID_ARG, fi + ": NULL scene or accumulation buffer");
+    if (rtg_status st = check_settings(ctx, fi, &J.settings, J.acc, J.s0, J.s1)) return st;
+    const uint32_t n_pix = (uint32_t)J.acc->w * (uint32_t)J.acc->h;
+    p0[i] = J.p1 == 0 ? 0 : J.p0;
+    p1[i] = J.p1 == 0 ? n_pix : J.p1;
+    if (p0[i] >= p1[i] || p1[i] > n_pix) return fail(ctx, RTG_ERR_INVALID_ARG, fi + ": bad pixel range");
+  }
+  for (uint32_t i = 0; i < n_jobs; ++i)
+    for (uint32_t j = i + 1; j < n_jobs; ++j)
+      if (jobs[i].acc == jobs[j].acc && p0[i] < p1[j] && p0[j] < p1[i])
+        return fail(ctx, RTG_ERR_INVALID_ARG,
+                    fn + ": jobs " + std::to_string(i) + " and " + std::to_string(j) +
+                        " write overlapping pixels of one accumulation buffer");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<rtg_stats> acc_stats(n_jobs, rtg_stats{});
+  // Jobs outside the batch kernels' scope, through the single-render path.
+  uint64_t batch_paths = 0;
+  for (uint32_t i = 0; i < n_jobs; ++i) {
+    if (batchable(jobs[i])) {
+      batch_paths += (uint64_t)(jobs[i].s1 - jobs[i].s0) * (p1[i] - p0[i]);
+      continue;
+    }
+    rtg_stats st{};
+    if (rtg_status e = render_range(ctx, jobs[i].scene, &jobs[i].settings, jobs[i].s0, jobs[i].s1, p0[i], p1[i],
+                                    jobs[i].acc, nullptr, &st, fn.c_str()))
+      return e;
+    acc_stats[i] = st;
+  }
+  if (batch_paths) {
+    const uint32_t cap = capacity_for(batch_paths);
+    if (rtg_status st = ensure_paths(ctx, kBatchBuffers, cap, ctx->pbb)) return st;
+    if (rtg_status st = ensure_batch_tables(ctx)) return st;
+    // Segments of at most `cap` paths (whole pixels x a sample sub-range),
+    // packed in job order into groups of at most `cap` paths.
+    std::vector<Segment> segs;
+    for (uint32_t i = 0; i < n_jobs; ++i) {
+      if (!batchable(jobs[i])) continue;
+      const uint32_t k_total = jobs[i].s1 - jobs[i].s0;
+      for (uint32_t sa = jobs[i].s0; sa < jobs[i].s1;) {
+        const uint32_t k = std::min(k_total - (sa - jobs[i].s0), cap);
+        const uint32_t per = std::max(1u, cap / k);
+        for (uint32_t p = p0[i]; p < p1[i]; p += per) segs.push_back(Segment{i, p, std::min(per, p1[i] - p), sa, k});
+        sa += k;
+      }
+    }
+    size_t a = 0;
+    while (a < segs.size()) {
+      size_t b = a;
+      uint64_t paths = 0;
+      while (b < segs.size() && b - a < (size_t)kMaxBatchJobs &&
+             paths + (uint64_t)segs[b].n_pix * segs[b].k <= cap) {
+        paths += (uint64_t)segs[b].n_pix * segs[b].k;
+        ++b;
+      }
+      if (rtg_status st = run_group(ctx, jobs, segs.data() + a, (uint32_t)(b - a), acc_stats)) return st;
+      a = b;
+    }
+  }
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats)
+    for (uint32_t i = 0; i < n_jobs; ++i) {
+      stats[i] = acc_stats[i];
+      stats[i].elapsed_ms = ms;
+    }
   return RTG_OK;
 }
 

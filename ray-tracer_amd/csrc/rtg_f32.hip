@@ -16,7 +16,7 @@ void chunk_impl(const DevScene& S, const PathBuffers<float>& B, const ChunkParam
   if (P.integrator != RTG_INTEGRATOR_PATH) {
     RTG_LAUNCH(h, RTG_K_DIRECT, (k_direct<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P);
   } else {
-    RTG_LAUNCH(h, RTG_K_GENERATE, k_generate<float>, dim3(grid_of(std::max<uint64_t>(P.n_paths, (uint64_t)(P.max_depth + 1) * kSubQueues))), dim3(kBlock), 0, s, S, B, P);
+    RTG_LAUNCH(h, RTG_K_GENERATE, k_generate<float>, dim3(grid_of(generate_threads(P))), dim3(kBlock), 0, s, S, B, P);
     const bool nee = P.light_sampling && S.n_lights > 0;
     for (int b = 0; b < P.max_depth; ++b) {
       RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
@@ -25,6 +25,29 @@ void chunk_impl(const DevScene& S, const PathBuffers<float>& B, const ChunkParam
     }
   }
   RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_accumulate<float>, dim3(grid_of(P.n_pix)), dim3(kBlock), 0, s, B, P);
+}
+
+template <int STACK>
+void batch_impl(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPlans& L, int depth, const BatchGrids& g,
+                hipStream_t s, const LaunchHooks* h) {
+  RTG_LAUNCH(h, RTG_K_GENERATE, k_batch_generate<float>, dim3(g.generate), dim3(kBlock), 0, s, jobs, B, L);
+  RTG_LAUNCH(h, RTG_K_PLAN, k_batch_plan<float>, dim3(1), dim3(kMaxBatchJobs), 0, s, jobs, L, -1);
+  for (int b = 0; b < depth; ++b) {
+    RTG_LAUNCH(h, RTG_K_EXTEND, (k_batch_extend<float, STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
+    RTG_LAUNCH(h, RTG_K_SHADE, k_batch_shade<float>, dim3(g.shade), dim3(kBlock), 0, s, jobs, B, L, b);
+    RTG_LAUNCH(h, RTG_K_PLAN, k_batch_plan<float>, dim3(1), dim3(kMaxBatchJobs), 0, s, jobs, L, b);
+    RTG_LAUNCH(h, RTG_K_SHADOW, (k_batch_shadow<float, STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
+  }
+  RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_batch_accumulate<float>, dim3(g.accumulate), dim3(kBlock), 0, s, jobs, B, L);
+}
+
+template <int STACK>
+uint32_t occupancy_trace() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_extend<float, STACK>, kBlock, 0) != hipSuccess) n = 1;
+  int m = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k_batch_shadow<float, STACK>, kBlock, 0) != hipSuccess) m = 1;
+  return (uint32_t)std::max(1, std::min(n, m));
 }
 
 template <int STACK>
@@ -40,6 +63,27 @@ void run_chunk(const DevScene& S, const PathBuffers<float>& B, const ChunkParams
   if (stack <= kStackClasses[0]) chunk_impl<20>(S, B, P, s, h);
   else if (stack <= kStackClasses[1]) chunk_impl<32>(S, B, P, s, h);
   else chunk_impl<64>(S, B, P, s, h);
+}
+
+void run_batch(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPlans& L, int depth, uint32_t stack,
+               const BatchGrids& g, hipStream_t s, const LaunchHooks* h) {
+  if (stack <= kStackClasses[0]) batch_impl<20>(jobs, B, L, depth, g, s, h);
+  else if (stack <= kStackClasses[1]) batch_impl<32>(jobs, B, L, depth, g, s, h);
+  else batch_impl<64>(jobs, B, L, depth, g, s, h);
+}
+
+void batch_occupancy(uint32_t stack, uint32_t* trace, uint32_t* shade, uint32_t* other) {
+  if (stack <= kStackClasses[0]) *trace = occupancy_trace<20>();
+  else if (stack <= kStackClasses[1]) *trace = occupancy_trace<32>();
+  else *trace = occupancy_trace<64>();
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_shade<float>, kBlock, 0) != hipSuccess) n = 1;
+  *shade = (uint32_t)std::max(1, n);
+  int m = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k_batch_generate<float>, kBlock, 0) != hipSuccess) m = 1;
+  int a = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_batch_accumulate<float>, kBlock, 0) != hipSuccess) a = 1;
+  *other = (uint32_t)std::max(1, std::min(m, a));
 }
 
 void run_trace32(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,

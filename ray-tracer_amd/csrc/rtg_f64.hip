@@ -50,7 +50,7 @@ void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParam
   if (P.integrator != RTG_INTEGRATOR_PATH) {
     RTG_LAUNCH(h, RTG_K_DIRECT, (k_direct<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P);
   } else {
-    RTG_LAUNCH(h, RTG_K_GENERATE, k_generate<double>, dim3(grid_of(std::max<uint64_t>(P.n_paths, (uint64_t)(P.max_depth + 1) * kSubQueues))), dim3(kBlock), 0, s, S, B, P);
+    RTG_LAUNCH(h, RTG_K_GENERATE, k_generate<double>, dim3(grid_of(generate_threads(P))), dim3(kBlock), 0, s, S, B, P);
     const bool nee = P.light_sampling && S.n_lights > 0;
     for (int b = 0; b < P.max_depth; ++b) {
       RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);

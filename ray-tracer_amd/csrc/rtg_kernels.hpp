@@ -54,6 +54,10 @@ __host__ __device__ constexpr uint32_t sub_queue_cap(uint32_t capacity) {
   return ((capacity + kBlock - 1) / kBlock + kSubQueues - 1) / kSubQueues * kBlock;
 }
 
+// One job of a pass: a range of pixels x a range of samples of one render.
+// A single render's chunk is one job at path id 0; a batch packs many jobs
+// into one set of path arrays, each with its own path-id range, queue
+// storage, queue counters and statistics slots.
 struct ChunkParams {
   uint32_t width, height;
   uint32_t p0, k, s0;  // pixels [p0, p0 + n_pix), samples [s0, s0 + k)
@@ -63,6 +67,14 @@ struct ChunkParams {
   uint64_t seed;
   double* acc_sum;      // width*height*3
   uint32_t* acc_count;  // width*height
+  uint32_t pid_base;       // first path id of the job
+  uint32_t q_base;         // first queue slot of the job
+  uint32_t sub_cap;        // entries per sub-queue
+  uint32_t shadow_counts;  // offset of the shadow-queue counters in `counts`
+  uint32_t stat_slots;     // statistics slots of the job
+  uint32_t zero_stats;     // generate clears the job's statistics slots
+  uint32_t* counts;        // queue sizes: [b*kSubQueues + s] active, [shadow_counts + b*kSubQueues + s] shadow
+  unsigned long long* stats;
 };
 
 // ---------------------------------------------------------------------------
@@ -78,7 +90,7 @@ struct AppendSlots {
   int64_t a, b;
 };
 __device__ __forceinline__ AppendSlots block_append2(uint32_t* counts_a, uint32_t* counts_b, uint32_t sub_cap,
-                                                     bool pa, bool pb) {
+                                                     bool pa, bool pb, uint32_t lb) {
   __shared__ uint32_t s_wave[2][kBlock / 64];
   __shared__ uint32_t s_base[2];
   const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -88,7 +100,7 @@ __device__ __forceinline__ AppendSlots block_append2(uint32_t* counts_a, uint32_
     s_wave[1][wave] = (uint32_t)__popcll(mb);
   }
   __syncthreads();
-  const uint32_t sub = blockIdx.x % kSubQueues;
+  const uint32_t sub = lb % kSubQueues;
   if (threadIdx.x < 2) {
     uint32_t tot = 0;
     for (uint32_t w = 0; w < kBlock / 64; ++w) tot += s_wave[threadIdx.x][w];
@@ -104,6 +116,7 @@ __device__ __forceinline__ AppendSlots block_append2(uint32_t* counts_a, uint32_
   AppendSlots r;
   r.a = pa ? (int64_t)sub * sub_cap + oa + lanes_below(ma) : -1;
   r.b = pb ? (int64_t)sub * sub_cap + ob + lanes_below(mb) : -1;
+  __syncthreads();  // s_wave / s_base are reused by the block's next job
   return r;
 }
 
@@ -134,9 +147,10 @@ __device__ __forceinline__ uint32_t queue_slot(const uint32_t* pre, uint32_t sub
   return (uint32_t)lo * sub_cap + (i - pre[lo]);
 }
 
-// Block-level reduction of per-thread counters into one of kStatSlots
-// slots (the host sums the slots).
-__device__ __forceinline__ void flush_stats(unsigned long long* g, const uint32_t* v, int n) {
+// Block-level reduction of per-thread counters into one of `slots` slots
+// (the host sums the slots).
+__device__ __forceinline__ void flush_stats(unsigned long long* g, const uint32_t* v, int n, uint32_t slots,
+                                            uint32_t lb) {
   __shared__ unsigned long long sh[kStCount];
   if (threadIdx.x < kStCount) sh[threadIdx.x] = 0;
   __syncthreads();
@@ -144,7 +158,11 @@ __device__ __forceinline__ void flush_stats(unsigned long long* g, const uint32_
     if (v[i]) atomicAdd(&sh[i], (unsigned long long)v[i]);
   __syncthreads();
   if (threadIdx.x < (unsigned)n && sh[threadIdx.x])
-    atomicAdd(&g[(blockIdx.x % kStatSlots) * kStatStride + threadIdx.x], sh[threadIdx.x]);
+    atomicAdd(&g[(lb % slots) * kStatStride + threadIdx.x], sh[threadIdx.x]);
+  __syncthreads();  // sh is reused by the block's next job
+}
+__device__ __forceinline__ void flush_stats(const ChunkParams& P, const uint32_t* v, uint32_t lb) {
+  flush_stats(P.stats, v, kStCount, P.stat_slots, lb);
 }
 
 template <class T>
@@ -234,26 +252,41 @@ __device__ __forceinline__ LdsStack lds_stack(uint32_t* ref, float* tn) {
 // traversal); sample-major puts neighbouring pixels of one sample together
 // (coalesced accumulation). Results do not depend on the layout.
 __device__ __forceinline__ uint32_t pid_pixel(const ChunkParams& P, uint32_t pid) {
-  return P.p0 + (P.sample_major ? pid % P.n_pix : pid / P.k);
+  const uint32_t l = pid - P.pid_base;
+  return P.p0 + (P.sample_major ? l % P.n_pix : l / P.k);
 }
 __device__ __forceinline__ uint32_t pid_sample(const ChunkParams& P, uint32_t pid) {
-  return P.s0 + (P.sample_major ? pid / P.n_pix : pid % P.k);
+  const uint32_t l = pid - P.pid_base;
+  return P.s0 + (P.sample_major ? l / P.n_pix : l % P.k);
 }
 __device__ __forceinline__ uint32_t pid_of(const ChunkParams& P, uint32_t j, uint32_t s) {
-  return P.sample_major ? s * P.n_pix + j : j * P.k + s;
+  return P.pid_base + (P.sample_major ? s * P.n_pix + j : j * P.k + s);
+}
+
+// Threads a job's generate step needs: one per path, and enough to clear
+// the job's queue counters (and statistics slots when asked).
+__host__ __device__ inline uint32_t generate_threads(const ChunkParams& P) {
+  uint32_t n = P.n_paths;
+  const uint32_t nz = (uint32_t)(P.max_depth + 1) * kSubQueues;
+  if (nz > n) n = nz;
+  if (P.zero_stats && P.stat_slots * kStatStride > n) n = P.stat_slots * kStatStride;
+  return n;
 }
 
 // generate: one camera ray per path (renderer.cpp:31-37). The bounce-0
-// queue is the identity and is not stored.
+// queue is the identity and is not stored. lb = the job's block index.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_generate(DevScene S, PathBuffers<T> B, ChunkParams P) {
-  const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void generate_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P,
+                                              uint32_t lb) {
+  const uint32_t l = lb * kBlock + threadIdx.x;
   const uint32_t nz = (uint32_t)(P.max_depth + 1) * kSubQueues;
-  if (pid < nz) {
-    B.counts[pid] = pid == 0 ? P.n_paths : 0u;
-    B.counts[kMaxBounce * kSubQueues + pid] = 0u;
+  if (l < nz) {
+    P.counts[l] = l == 0 ? P.n_paths : 0u;
+    P.counts[P.shadow_counts + l] = 0u;
   }
-  if (pid >= P.n_paths) return;
+  if (P.zero_stats && l < P.stat_slots * kStatStride) P.stats[l] = 0;
+  if (l >= P.n_paths) return;
+  const uint32_t pid = P.pid_base + l;
   const uint32_t pixel = pid_pixel(P, pid);
   const uint32_t sample = pid_sample(P, pid);
   const int x = (int)(pixel % P.width), y = (int)(pixel / P.width);
@@ -278,24 +311,27 @@ __global__ __launch_bounds__(kBlock) void k_generate(DevScene S, PathBuffers<T> 
   B.lx[pid] = 0; B.ly[pid] = 0; B.lz[pid] = 0;
   B.spec[pid] = 1;
 }
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_generate(DevScene S, PathBuffers<T> B, ChunkParams P) {
+  generate_body<T>(S, B, P, blockIdx.x);
+}
 
 // Path id of active entry i at bounce b (identity at bounce 0).
 template <class T>
-__device__ __forceinline__ uint32_t active_pid(const PathBuffers<T>& B, const uint32_t* pre, int b, uint32_t i) {
-  return b == 0 ? i : B.q[b & 1][queue_slot(pre, B.sub_cap, i)];
+__device__ __forceinline__ uint32_t active_pid(const PathBuffers<T>& B, const ChunkParams& P, const uint32_t* pre,
+                                               int b, uint32_t i) {
+  return b == 0 ? P.pid_base + i : B.q[b & 1][P.q_base + queue_slot(pre, P.sub_cap, i)];
 }
 
 // extend: closest hit for the active queue of bounce b.
 template <class T, int STACK>
-__global__ __launch_bounds__(kBlock) void k_extend(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
-  __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
-  __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
-  __shared__ uint32_t pre[kSubQueues + 1];
-  const QueueView qv = load_queue(B.counts + (size_t)b * kSubQueues, pre);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void extend_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P, int b,
+                                            uint32_t lb, uint32_t* sref, float* stn, uint32_t* pre) {
+  const QueueView qv = load_queue(P.counts + (size_t)b * kSubQueues, pre);
+  const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
   if (i < qv.total) {
-    const uint32_t pid = active_pid(B, pre, b, i);
+    const uint32_t pid = active_pid(B, P, pre, b, i);
     const V3<T> o{B.ox[pid], B.oy[pid], B.oz[pid]}, d{B.dx[pid], B.dy[pid], B.dz[pid]};
     Counters c{0, 0};
     const auto h = ModeOps<T>::closest(S, P.use_bvh != 0, o, d, lds_stack<STACK>(sref, stn), c);
@@ -308,21 +344,28 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene S, PathBuffers<T> B,
     st[kStVisits] = c.visits;
     st[kStTests] = c.tests;
   }
-  flush_stats(B.stats, st, kStCount);
+  flush_stats(P, st, lb);
+}
+template <class T, int STACK>
+__global__ __launch_bounds__(kBlock) void k_extend(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
+  __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
+  __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
+  __shared__ uint32_t pre[kSubQueues + 1];
+  extend_body<T, STACK>(S, B, P, b, blockIdx.x, sref, stn, pre);
 }
 
 // shade: one bounce of path_trace (integrator.cpp:103-137) after the hit.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
-  __shared__ uint32_t pre[kSubQueues + 1];
-  const QueueView qv = load_queue(B.counts + (size_t)b * kSubQueues, pre);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P, int b,
+                                           uint32_t lb, uint32_t* pre) {
+  const QueueView qv = load_queue(P.counts + (size_t)b * kSubQueues, pre);
+  const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
   bool alive = false, shadow = false;
   uint32_t pid = 0;
   ShadowReq<T> req;
   if (i < qv.total) {
-    pid = active_pid(B, pre, b, i);
+    pid = active_pid(B, P, pre, b, i);
     const uint32_t pixel = pid_pixel(P, pid), sample = pid_sample(P, pid);
     const V3<T> o{B.ox[pid], B.oy[pid], B.oz[pid]}, d{B.dx[pid], B.dy[pid], B.dz[pid]};
     V3<T> beta{B.bx[pid], B.by[pid], B.bz[pid]};
@@ -376,11 +419,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, PathBuffers<T> B, 
     B.lx[pid] = L.x; B.ly[pid] = L.y; B.lz[pid] = L.z;
   }
   // Converged: compacted appends to the next active queue and the shadow queue.
-  const AppendSlots a = block_append2(B.counts + (size_t)(b + 1) * kSubQueues,
-                                      B.counts + (size_t)(kMaxBounce + b) * kSubQueues, B.sub_cap, alive, shadow);
-  if (alive) B.q[(b + 1) & 1][a.a] = pid;
+  const AppendSlots a = block_append2(P.counts + (size_t)(b + 1) * kSubQueues,
+                                      P.counts + P.shadow_counts + (size_t)b * kSubQueues, P.sub_cap, alive, shadow,
+                                      lb);
+  if (alive) B.q[(b + 1) & 1][P.q_base + a.a] = pid;
   if (shadow) {
-    const int64_t si = a.b;
+    const int64_t si = P.q_base + a.b;
     B.sq_pid[si] = pid;
     B.sox[si] = req.o.x; B.soy[si] = req.o.y; B.soz[si] = req.o.z;
     B.sdx[si] = req.d.x; B.sdy[si] = req.d.y; B.sdz[si] = req.d.z;
@@ -388,20 +432,23 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, PathBuffers<T> B, 
     B.scx[si] = req.f_rop.x; B.scy[si] = req.f_rop.y; B.scz[si] = req.f_rop.z;
     st[kStShadow] = 1;
   }
-  flush_stats(B.stats, st, kStCount);
+  flush_stats(P, st, lb);
+}
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
+  __shared__ uint32_t pre[kSubQueues + 1];
+  shade_body<T>(S, B, P, b, blockIdx.x, pre);
 }
 
 // shadow: any-hit of the shadow queue of bounce b (Scene::occluded).
 template <class T, int STACK>
-__global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
-  __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
-  __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
-  __shared__ uint32_t pre[kSubQueues + 1];
-  const QueueView qv = load_queue(B.counts + (size_t)(kMaxBounce + b) * kSubQueues, pre);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void shadow_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P, int b,
+                                            uint32_t lb, uint32_t* sref, float* stn, uint32_t* pre) {
+  const QueueView qv = load_queue(P.counts + P.shadow_counts + (size_t)b * kSubQueues, pre);
+  const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
   if (i < qv.total) {
-    const uint32_t k = queue_slot(pre, B.sub_cap, i);
+    const uint32_t k = P.q_base + queue_slot(pre, P.sub_cap, i);
     const uint32_t pid = B.sq_pid[k];
     const V3<T> o{B.sox[k], B.soy[k], B.soz[k]}, d{B.sdx[k], B.sdy[k], B.sdz[k]};
     Counters c{0, 0};
@@ -414,14 +461,21 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, PathBuffers<T> B,
     st[kStSVisits] = c.visits;
     st[kStSTests] = c.tests;
   }
-  flush_stats(B.stats, st, kStCount);
+  flush_stats(P, st, lb);
+}
+template <class T, int STACK>
+__global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
+  __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
+  __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
+  __shared__ uint32_t pre[kSubQueues + 1];
+  shadow_body<T, STACK>(S, B, P, b, blockIdx.x, sref, stn, pre);
 }
 
 // accumulate: the invalid-sample guard and Framebuffer::add in sample order
 // (renderer.cpp:39-45, framebuffer.hpp:31-35).
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_accumulate(PathBuffers<T> B, ChunkParams P) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void accumulate_body(const PathBuffers<T>& B, const ChunkParams& P, uint32_t lb) {
+  const uint32_t j = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
   if (j < P.n_pix) {
     const uint32_t pixel = P.p0 + j;
@@ -444,7 +498,11 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(PathBuffers<T> B, ChunkPa
     P.acc_count[pixel] += P.k;
     st[kStCamera] = P.k;
   }
-  flush_stats(B.stats, st, kStCount);
+  flush_stats(P, st, lb);
+}
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_accumulate(PathBuffers<T> B, ChunkParams P) {
+  accumulate_body<T>(B, P, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -586,9 +644,10 @@ template <class T, int STACK>
 __global__ __launch_bounds__(kBlock) void k_direct(DevScene S, PathBuffers<T> B, ChunkParams P) {
   __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
   __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
-  const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t pid = P.pid_base + l;
   uint32_t st[kStCount] = {0};
-  if (pid < P.n_paths) {
+  if (l < P.n_paths) {
     const uint32_t pixel = pid_pixel(P, pid), sample = pid_sample(P, pid);
     const int x = (int)(pixel % P.width), y = (int)(pixel / P.width);
     V3<T> o, d, L;
@@ -612,7 +671,7 @@ __global__ __launch_bounds__(kBlock) void k_direct(DevScene S, PathBuffers<T> B,
     B.ly[pid] = L.y;
     B.lz[pid] = L.z;
   }
-  flush_stats(B.stats, st, kStCount);
+  flush_stats(P, st, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -668,7 +727,139 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const double* __re
     st[kStVisits] = c.visits;
     st[kStTests] = c.tests;
   }
-  flush_stats(stats, st, kStCount);
+  flush_stats(stats, st, kStCount, kStatSlots, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batches: many jobs (different scenes, resolutions, sample ranges, depths)
+// share one set of path arrays and one launch per step. A plan holds, per
+// job, the first virtual block of the job (prefix sums, n_jobs + 1 entries).
+// Workgroups are persistent and walk the virtual blocks round-robin, so each
+// virtual block belongs to one job: the job's scene and settings are uniform
+// across the workgroup and sit in scalar registers.
+struct BatchJob {
+  DevScene S;
+  ChunkParams P;
+};
+constexpr int kMaxBatchJobs = 256;
+constexpr int kBatchDepth = 64;                                 // max_depth limit of batched jobs
+constexpr uint32_t kBatchShadowCounts = (kBatchDepth + 1) * kSubQueues;
+constexpr uint32_t kBatchCounts = 2 * kBatchShadowCounts;       // counters per job
+constexpr uint32_t kBatchStatSlots = 32;
+
+struct BatchPlans {
+  const uint32_t* gen;  // written by the host
+  const uint32_t* acc;  // written by the host
+  uint32_t* act;        // active queue of the next extend/shade (k_batch_plan)
+  uint32_t* shd;        // shadow queue of the next shadow step (k_batch_plan)
+  uint32_t n_jobs;
+};
+
+__device__ __forceinline__ uint32_t blocks_of(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+// Job of virtual block vb: the last j with plan[j] <= vb (jobs without
+// blocks repeat their successor's start and are skipped).
+__device__ __forceinline__ uint32_t job_of(const uint32_t* plan, uint32_t n_jobs, uint32_t vb) {
+  int lo = 0, hi = (int)n_jobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (plan[mid] <= vb) lo = mid;
+    else hi = mid - 1;
+  }
+  return (uint32_t)__builtin_amdgcn_readfirstlane(lo);
+}
+
+// Runs f(job, local block) over every virtual block of a plan.
+template <class F>
+__device__ __forceinline__ void batch_loop(const uint32_t* __restrict__ plan_g, uint32_t n_jobs, uint32_t* s_plan,
+                                           F&& f) {
+  for (uint32_t t = threadIdx.x; t <= n_jobs; t += blockDim.x) s_plan[t] = plan_g[t];
+  __syncthreads();
+  const uint32_t total = s_plan[n_jobs];
+  for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const uint32_t j = job_of(s_plan, n_jobs, vb);
+    f(j, vb - s_plan[j]);
+  }
+}
+
+// Plan of the active queue of bounce b + 1 and of the shadow queue of
+// bounce b (b = -1: the camera rays of bounce 0 only). One workgroup.
+template <class T>
+__global__ __launch_bounds__(kMaxBatchJobs) void k_batch_plan(const BatchJob* __restrict__ jobs, BatchPlans L, int b) {
+  __shared__ uint32_t sa[kMaxBatchJobs], ss[kMaxBatchJobs];
+  const uint32_t j = threadIdx.x;
+  uint32_t na = 0, ns = 0;
+  if (j < L.n_jobs) {
+    const ChunkParams& P = jobs[j].P;
+    if (b + 1 < P.max_depth)
+      for (int s = 0; s < kSubQueues; ++s) na += P.counts[(size_t)(b + 1) * kSubQueues + s];
+    if (b >= 0 && b < P.max_depth)
+      for (int s = 0; s < kSubQueues; ++s) ns += P.counts[P.shadow_counts + (size_t)b * kSubQueues + s];
+  }
+  sa[j] = blocks_of(na);
+  ss[j] = blocks_of(ns);
+  __syncthreads();
+  for (uint32_t off = 1; off < kMaxBatchJobs; off <<= 1) {  // inclusive scan
+    const uint32_t a = j >= off ? sa[j - off] : 0u, c = j >= off ? ss[j - off] : 0u;
+    __syncthreads();
+    sa[j] += a;
+    ss[j] += c;
+    __syncthreads();
+  }
+  if (j < L.n_jobs) {
+    L.act[j + 1] = sa[j];
+    L.shd[j + 1] = ss[j];
+  }
+  if (j == 0) {
+    L.act[0] = 0;
+    L.shd[0] = 0;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_batch_generate(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
+                                                           BatchPlans L) {
+  __shared__ uint32_t s_plan[kMaxBatchJobs + 1];
+  batch_loop(L.gen, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) { generate_body<T>(jobs[j].S, B, jobs[j].P, lb); });
+}
+
+template <class T, int STACK>
+__global__ __launch_bounds__(kBlock) void k_batch_extend(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
+                                                         BatchPlans L, int b) {
+  __shared__ uint32_t sref[STACK * kBlock];
+  __shared__ float stn[STACK * kBlock];
+  __shared__ uint32_t pre[kSubQueues + 1];
+  __shared__ uint32_t s_plan[kMaxBatchJobs + 1];
+  batch_loop(L.act, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) {
+    extend_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, stn, pre);
+  });
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_batch_shade(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
+                                                        BatchPlans L, int b) {
+  __shared__ uint32_t pre[kSubQueues + 1];
+  __shared__ uint32_t s_plan[kMaxBatchJobs + 1];
+  batch_loop(L.act, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) { shade_body<T>(jobs[j].S, B, jobs[j].P, b, lb, pre); });
+}
+
+template <class T, int STACK>
+__global__ __launch_bounds__(kBlock) void k_batch_shadow(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
+                                                         BatchPlans L, int b) {
+  __shared__ uint32_t sref[STACK * kBlock];
+  __shared__ float stn[STACK * kBlock];
+  __shared__ uint32_t pre[kSubQueues + 1];
+  __shared__ uint32_t s_plan[kMaxBatchJobs + 1];
+  batch_loop(L.shd, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) {
+    shadow_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, stn, pre);
+  });
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_batch_accumulate(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
+                                                             BatchPlans L) {
+  __shared__ uint32_t s_plan[kMaxBatchJobs + 1];
+  batch_loop(L.acc, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) { accumulate_body<T>(B, jobs[j].P, lb); });
 }
 
 }  // namespace rtg

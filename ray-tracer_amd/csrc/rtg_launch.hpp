@@ -30,6 +30,17 @@ void run_chunk(const DevScene& S, const PathBuffers<float>& B, const ChunkParams
 void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
                const LaunchHooks* hooks);
 
+// Grid sizes of one batched pass (persistent workgroups; see k_batch_*).
+struct BatchGrids {
+  uint32_t generate, trace, shade, accumulate;
+};
+// One batched pass (FP32 path integrator): generate, max_depth x (extend,
+// shade, plan, shadow), accumulate over the jobs of `jobs` (device table).
+void run_batch(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPlans& L, int depth, uint32_t stack,
+               const BatchGrids& g, hipStream_t s, const LaunchHooks* hooks);
+// Resident workgroups per CU of the batch kernels (for the persistent grids).
+void batch_occupancy(uint32_t stack, uint32_t* trace, uint32_t* shade, uint32_t* other);
+
 void run_trace32(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,
                  unsigned long long* stats, uint32_t stack, hipStream_t s, const LaunchHooks* hooks);
 void run_trace64(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,

@@ -83,6 +83,12 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Job(C.Structure):
+    """rtg_job: one pass (samples [s0, s1) of pixels [p0, p1)) of one render."""
+    _fields_ = [("scene", C.c_void_p), ("acc", C.c_void_p), ("settings", RenderSettings),
+                ("s0", C.c_uint32), ("s1", C.c_uint32), ("p0", C.c_uint32), ("p1", C.c_uint32)]
+
+
 class Hit(C.Structure):
     _fields_ = [("t", C.c_double), ("prim", C.c_uint32), ("flags", C.c_uint32)]
 
@@ -137,6 +143,7 @@ def load_rtg():
         "rtg_accum_read": (C.c_int32, [_VP, _VP, _VP, _VP]),
         "rtg_accum_min_samples": (C.c_uint32, [_VP, _VP]),
         "rtg_render_pass": (C.c_int32, [_VP, _VP, P(RenderSettings), C.c_uint32, C.c_uint32, _VP, _VP, P(Stats)]),
+        "rtg_render_batch": (C.c_int32, [_VP, P(Job), C.c_uint32, P(Stats)]),
         "rtg_resolve": (C.c_int32, [_VP, _VP, C.c_int32, C.c_double, _VP]),
         "rtg_buffer_create": (C.c_int32, [_VP, C.c_uint64, P(_VP)]),
         "rtg_buffer_release": (C.c_int32, [_VP]),
@@ -157,7 +164,8 @@ def rtg_symbols():
     """Names declared in include/rtg.h (checked by the CPU tests)."""
     return ["rtg_open", "rtg_close", "rtg_last_error", "rtg_abi_version", "rtg_device_info", "rtg_scene_create",
             "rtg_scene_release", "rtg_scene_bytes", "rtg_accum_create", "rtg_accum_release", "rtg_accum_clear",
-            "rtg_accum_read", "rtg_accum_min_samples", "rtg_render_pass", "rtg_resolve", "rtg_buffer_create",
+            "rtg_accum_read", "rtg_accum_min_samples", "rtg_render_pass", "rtg_render_batch", "rtg_resolve",
+            "rtg_buffer_create",
             "rtg_buffer_release", "rtg_buffer_write", "rtg_buffer_read", "rtg_trace", "rtg_trace_host",
             "rtg_synchronize", "rtg_set_profiling", "rtg_get_profile"]
 
@@ -215,6 +223,18 @@ class Context:
             return st, True
         self._check(rc)
         return st, False
+
+    def render_batch(self, jobs):
+        """jobs: list of (scene, accum, settings, s0, s1[, p0, p1]). Returns
+        one Stats per job (rtg_render_batch)."""
+        arr = (Job * max(1, len(jobs)))()
+        for i, j in enumerate(jobs):
+            scene, acc, cfg, s0, s1 = j[:5]
+            p0, p1 = (j[5], j[6]) if len(j) > 5 else (0, 0)
+            arr[i] = Job(scene.h.value, acc.h.value, cfg, s0, s1, p0, p1)
+        st = (Stats * max(1, len(jobs)))()
+        self._check(self.lib.rtg_render_batch(self.h, arr, len(jobs), st))
+        return list(st)[:len(jobs)]
 
     def trace(self, scene, rays: np.ndarray, mode=0, use_bvh=1, precision=PRECISION_F64):
         rays = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 8)
@@ -444,11 +464,11 @@ def validate(text):
     return load_host().rt_validate(text.encode()).decode()
 
 
-K_NAMES = ["generate", "extend", "shade", "shadow", "accumulate", "direct", "trace"]
+K_NAMES = ["generate", "extend", "shade", "shadow", "accumulate", "direct", "trace", "plan"]
 
 
 class Profile(C.Structure):
-    _fields_ = [("launches", C.c_uint64 * 7), ("ms", C.c_double * 7),
+    _fields_ = [("launches", C.c_uint64 * 8), ("ms", C.c_double * 8),
                 ("closest_rays", C.c_uint64), ("closest_visits", C.c_uint64), ("closest_tests", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("shadow_visits", C.c_uint64), ("shadow_tests", C.c_uint64)]
 

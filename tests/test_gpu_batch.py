@@ -1,0 +1,145 @@
+"""rtg_render_batch: many jobs (scenes, resolutions, sample ranges, depths)
+in shared launches. A job's samples are keyed on (seed, pixel, sample,
+bounce) and its sums are added in sample order, so every batched job must
+equal the same pass rendered alone with rtg_render_pass bit for bit, and its
+counters (rays, shadow rays, roulette, node visits, primitive tests) must be
+identical."""
+import os
+
+import numpy as np
+import pytest
+
+import orc
+import scenes
+from _pkg import rtg
+
+pytestmark = pytest.mark.gpu
+COUNTERS = ("camera_samples", "rays", "shadow_rays", "rr_terminated", "invalid_samples", "node_visits", "prim_tests")
+
+
+def job_spec(j, w, h, spp, seed=None, depth=None, precision=rtg.PRECISION_F32, integrator=None):
+    over = {"width": w, "height": h}
+    if seed is not None:
+        over["seed"] = seed
+    if integrator is not None:
+        over["integrator"] = integrator
+    o = orc.OracleScene(j, **over)
+    cfg = o.settings(precision)
+    if depth is not None:
+        cfg.max_depth = depth
+    return o, cfg, spp
+
+
+@pytest.fixture(scope="module")
+def specs():
+    return [
+        job_spec(scenes.load("cornell_c1.json"), 32, 32, 4),
+        job_spec(scenes.load("torus_c2.json"), 24, 20, 3),
+        job_spec(scenes.load("sphere_field_c3.json"), 40, 24, 2),
+        job_spec(scenes.load_asset("materials.json"), 24, 12, 5),
+        job_spec(scenes.load_asset("cornell_box.json"), 17, 13, 7, seed=5, depth=3),
+        job_spec(scenes.load_asset("first_light.json"), 16, 16, 1),
+        job_spec(scenes.load_asset("cornell_glass_metal.json"), 20, 20, 2, depth=12),
+    ]
+
+
+def solo(ctx, o, cfg, s0, s1, p=None):
+    dev = ctx.scene(o.desc)
+    acc = ctx.accum(cfg.width, cfg.height)
+    st, _ = ctx.render_pass(dev, cfg, s0, s1, acc)
+    s, c = acc.read()
+    acc.release()
+    dev.release()
+    return s, c, st
+
+
+def batched(ctx, specs, ranges):
+    devs = [ctx.scene(o.desc) for o, _, _ in specs]
+    accs = [ctx.accum(cfg.width, cfg.height) for _, cfg, _ in specs]
+    out = []
+    for s0, s1 in ranges:
+        jobs = [(d, a, cfg, s0, min(s1, spp)) for d, a, (o, cfg, spp) in zip(devs, accs, specs) if s0 < spp]
+        out.append(ctx.render_batch(jobs))
+    res = [a.read() for a in accs]
+    for a in accs:
+        a.release()
+    for d in devs:
+        d.release()
+    return res, out
+
+
+def test_batch_equals_solo_renders(ctx, specs):
+    (res, (stats,)) = batched(ctx, specs, [(0, 1000)])
+    for (o, cfg, spp), (s, c), st in zip(specs, res, stats):
+        s1, c1, st1 = solo(ctx, o, cfg, 0, spp)
+        assert np.array_equal(c, c1)
+        assert np.array_equal(s.view(np.uint64), s1.view(np.uint64))
+        for k in COUNTERS:
+            assert getattr(st, k) == getattr(st1, k), k
+
+
+def test_progressive_batches_equal_one_solo_pass(ctx, specs):
+    """Passes 1, 1, 2, ... of every job batched together = one solo pass."""
+    res, _ = batched(ctx, specs, [(0, 1), (1, 2), (2, 4), (4, 8)])
+    for (o, cfg, spp), (s, c) in zip(specs, res):
+        s1, c1, _ = solo(ctx, o, cfg, 0, spp)
+        assert np.array_equal(c, c1) and np.array_equal(s, s1)
+
+
+def test_pixel_ranges_of_one_buffer(ctx):
+    o, cfg, _ = job_spec(scenes.load("cornell_c1.json"), 30, 20, 3)
+    dev = ctx.scene(o.desc)
+    acc = ctx.accum(30, 20)
+    st = ctx.render_batch([(dev, acc, cfg, 0, 3, 0, 7), (dev, acc, cfg, 0, 3, 7, 450), (dev, acc, cfg, 0, 3, 450, 600)])
+    s, c = acc.read()
+    with pytest.raises(rtg.RtgError, match="overlapping"):
+        ctx.render_batch([(dev, acc, cfg, 3, 4, 0, 10), (dev, acc, cfg, 3, 4, 9, 20)])
+    acc.release()
+    dev.release()
+    s1, c1, st1 = solo(ctx, o, cfg, 0, 3)
+    assert np.array_equal(c, c1) and np.array_equal(s, s1)
+    assert sum(x.rays for x in st) == st1.rays
+
+
+def test_jobs_outside_the_batch_kernels(ctx):
+    """FP64 and direct/normals jobs go through the single-render path."""
+    sp = [job_spec(scenes.load_asset("materials.json"), 20, 12, 2, precision=rtg.PRECISION_F64),
+          job_spec(scenes.load_asset("materials.json"), 20, 12, 2, integrator="direct"),
+          job_spec(scenes.load_asset("mirrors.json"), 16, 8, 2, integrator="normals"),
+          job_spec(scenes.load("cornell_c1.json"), 16, 16, 2)]
+    res, (stats,) = batched(ctx, sp, [(0, 2)])
+    for (o, cfg, spp), (s, c), st in zip(sp, res, stats):
+        s1, c1, st1 = solo(ctx, o, cfg, 0, spp)
+        assert np.array_equal(c, c1) and np.array_equal(s, s1)
+        assert st.rays == st1.rays
+
+
+def test_many_small_jobs_in_several_groups(ctx, specs, monkeypatch):
+    """300 jobs (more than one group holds) and a small capacity that splits
+    jobs into segments of whole pixels and sample sub-ranges."""
+    monkeypatch.setenv("RTG_CHUNK_PATHS", "4096")
+    pool = [specs[i][0].json for i in (0, 2, 3, 4, 5, 6)]  # the small scenes
+    many = [job_spec(pool[i % len(pool)], 8 + i % 5, 6 + i % 3, 1 + i % 3, seed=i) for i in range(300)]
+    many[7] = job_spec(scenes.load("cornell_c1.json"), 64, 48, 3)  # 9216 paths: split into segments
+    res, (stats,) = batched(ctx, many, [(0, 3)])
+    for i in (0, 1, 7, 123, 299):
+        o, cfg, spp = many[i]
+        s1, c1, st1 = solo(ctx, o, cfg, 0, spp)
+        assert np.array_equal(res[i][1], c1) and np.array_equal(res[i][0], s1), i
+        assert stats[i].rays == st1.rays and stats[i].node_visits == st1.node_visits
+
+
+def test_bad_jobs_are_rejected(ctx, specs):
+    o, cfg, _ = specs[0]
+    dev = ctx.scene(o.desc)
+    acc = ctx.accum(cfg.width, cfg.height)
+    with pytest.raises(rtg.RtgError, match="empty sample range"):
+        ctx.render_batch([(dev, acc, cfg, 2, 2)])
+    with pytest.raises(rtg.RtgError, match="bad pixel range"):
+        ctx.render_batch([(dev, acc, cfg, 0, 1, 5, cfg.width * cfg.height + 1)])
+    small = ctx.accum(4, 4)
+    with pytest.raises(rtg.RtgError, match="size differs"):
+        ctx.render_batch([(dev, small, cfg, 0, 1)])
+    small.release()
+    acc.release()
+    dev.release()
