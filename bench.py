@@ -87,6 +87,143 @@ def cpu_baseline(scene_text, width, height, target_s=12.0):
             "sample": f"{width}x{height} at {spp} spp (of 64), progressive passes, {rays} rays in {secs:.2f} s"}
 
 
+def pass_schedule(spp, max_pass=32):
+    """Progressive passes (renderer.cpp:66-69): 1, 1, 2, 4, ... capped."""
+    out, done = [], 0
+    while done < spp:
+        k = min(spp - done, max(1, done), max_pass)
+        out.append((done, done + k))
+        done += k
+    return out
+
+
+def batch_leg(rtg, seed=4, count=64):
+    """The batched workload (scenes/make_batch.py): `count` complete renders
+    from one seed, each with the reference's progressive pass schedule.
+    Plain: one render after another, each pass through rtg_render_pass.
+    Batched: pass i of every unfinished render in one rtg_render_batch call.
+    The images of both are compared bitwise."""
+    import hashlib
+    sys.path.insert(0, os.path.join(PKG, "scenes"))
+    import make_batch
+    specs = make_batch.build(seed, count)
+    t0 = time.perf_counter()
+    hs = [rtg.HostScene(json.dumps(j)) for j in specs]
+    prep_s = time.perf_counter() - t0
+    ctx = rtg.Context(0)
+    devs = [ctx.scene(h.desc) for h in hs]
+    cfgs = [h.settings(rtg.PRECISION_F32) for h in hs]
+    scheds = [pass_schedule(int(h.info()["spp"])) for h in hs]
+    accs = [ctx.accum(c.width, c.height) for c in cfgs]
+    big = max(range(count), key=lambda i: cfgs[i].width * cfgs[i].height)
+    ctx.render_pass(devs[big], cfgs[big], 0, 32, accs[big])  # grow the path buffers before timing
+    ctx.render_batch([(devs[big], accs[big], cfgs[big], 0, 32)])
+    for a in accs:
+        a.clear()
+    ctx.synchronize()
+
+    def digests():
+        return [hashlib.sha256(b"".join(x.tobytes() for x in a.read())).hexdigest() for a in accs]
+
+    t0 = time.perf_counter()
+    rays_p = 0
+    for d, a, c, sched in zip(devs, accs, cfgs, scheds):
+        for s0, s1 in sched:
+            st, _ = ctx.render_pass(d, c, s0, s1, a)
+            rays_p += st.rays + st.shadow_rays
+    ctx.synchronize()
+    plain_s = time.perf_counter() - t0
+    ref = digests()
+    for a in accs:
+        a.clear()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    rays_b, calls = 0, 0
+    for p in range(max(len(s) for s in scheds)):
+        jobs = [(devs[i], accs[i], cfgs[i]) + scheds[i][p] for i in range(count) if p < len(scheds[i])]
+        rays_b += sum(st.rays + st.shadow_rays for st in ctx.render_batch(jobs))
+        calls += 1
+    ctx.synchronize()
+    batched_s = time.perf_counter() - t0
+    identical = digests() == ref
+    for x in accs + devs:
+        x.release()
+    ctx.close()
+    kinds = {}
+    for j in specs:
+        k = j["name"].split()[0]
+        kinds[k] = kinds.get(k, 0) + 1
+    return {"workload": f"{count} renders from seed {seed}: " + ", ".join(f"{v} {k}" for k, v in sorted(kinds.items())) +
+                        "; 64^2-1024^2, 1-256 spp, progressive passes",
+            "paths": int(sum(c.width * c.height * int(h.info()["spp"]) for c, h in zip(cfgs, hs))),
+            "rays": int(rays_b), "host_prep_s": round(prep_s, 2),
+            "plain_s": round(plain_s, 4), "batched_s": round(batched_s, 4), "batch_calls": calls,
+            "plain_g_rays_per_s": rays_p / plain_s / 1e9, "batched_g_rays_per_s": rays_b / batched_s / 1e9,
+            "speedup": plain_s / batched_s, "identical": identical and rays_p == rays_b}
+
+
+def serve_leg(rtg, qps=6.0, seconds=10.0, seed=5, burst=300):
+    """The served workload: Poisson arrivals at `qps` for `seconds` over 16
+    resident scenes of the batched workload. 80 % previews (128^2, 1-4 spp,
+    depth 4, one pass), 20 % finals (512^2, 768^2 or 1024^2, 256 spp, the
+    scene's depth, progressive passes capped at 32). The same trace runs
+    through the served loop and the plain loop; then `burst` previews arriving
+    at once measure saturation throughput."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(PKG, "scenes"))
+    import make_batch
+    specs = make_batch.build(seed, 16)
+    hs = [rtg.HostScene(json.dumps(j)) for j in specs]
+    depth = [int(j["render"]["max_depth"]) for j in specs]
+    srv = rtg.Server(0, 0, 0)
+    for h in hs:
+        srv.add_scene(h)
+    rng = np.random.default_rng(seed)
+
+    def requests(n, arrivals, final):
+        r = np.zeros(n, rtg.REQUEST_DTYPE)
+        r["scene"] = rng.integers(0, 16, n)
+        r["seed"] = rng.integers(1, 10 ** 9, n)
+        r["arrival_ms"] = arrivals
+        r["final_kind"] = final
+        size = rng.choice([512, 768, 1024], n)
+        r["width"] = r["height"] = np.where(final, size, 128)
+        r["spp"] = np.where(final, 256, rng.integers(1, 5, n))
+        r["max_depth"] = np.where(final, np.array(depth)[r["scene"]], 4)
+        r["max_pass"] = np.where(final, 32, 4)
+        return r
+
+    warm = requests(4, np.zeros(4), np.array([1, 0, 0, 0]))
+    warm["width"][0] = warm["height"][0] = 1024
+    srv.run(warm, served=True)  # grows the path buffers before timing
+    srv.run(warm, served=False)
+    n = int(rng.poisson(qps * seconds))
+    reqs = requests(n, np.sort(rng.uniform(0.0, seconds * 1e3, n)), rng.random(n) < 0.2)
+    out = {"trace": f"{n} requests in {seconds:.0f} s (Poisson, {qps:g}/s): {int(reqs['final_kind'].sum())} finals, "
+                    f"{int((reqs['final_kind'] == 0).sum())} previews over 16 resident scenes"}
+    res = {}
+    for name, served in (("served", True), ("plain", False)):
+        r, st = srv.run(reqs, served=served)
+        lat = r["done_ms"] - r["arrival_ms"]
+        prev, fin = reqs["final_kind"] == 0, reqs["final_kind"] == 1
+        res[name] = r
+        out[name] = {"preview_p50_ms": float(np.percentile(lat[prev], 50)),
+                     "preview_p99_ms": float(np.percentile(lat[prev], 99)),
+                     "final_p50_ms": float(np.percentile(lat[fin], 50)) if fin.any() else None,
+                     "final_first_frame_p50_ms": float(np.percentile((r["first_frame_ms"] - r["arrival_ms"])[fin], 50))
+                     if fin.any() else None,
+                     "wall_s": st["wall_ms"] / 1e3, "g_rays_per_s": st["rays"] / st["wall_ms"] / 1e6,
+                     "ticks": st["ticks"], "max_active": st["max_active"]}
+    out["identical"] = bool(np.array_equal(res["served"]["image_hash"], res["plain"]["image_hash"]))
+    pb = requests(burst, np.zeros(burst), np.zeros(burst, bool))
+    for name, served in (("served", True), ("plain", False)):
+        r, st = srv.run(pb, served=served)
+        out[name]["burst_previews_per_s"] = burst / (st["wall_ms"] / 1e3)
+    out["preview_throughput_ratio"] = out["served"]["burst_previews_per_s"] / out["plain"]["burst_previews_per_s"]
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +232,10 @@ def main():
     ap.add_argument("--workload", default="torus_c2", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
+    ap.add_argument("--qps", type=float, default=6.0, help="served workload: request arrival rate")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="served workload: trace length")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (3 times): the reference kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -163,6 +304,8 @@ def main():
     dev.release()
     acc.release()
     ctx.close()
+    batch = None if args.no_batch else batch_leg(rtg)
+    served = None if args.no_serve else serve_leg(rtg, qps=args.qps, seconds=args.serve_seconds)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(text, cfg.width, cfg.height)
@@ -178,6 +321,8 @@ def main():
                    "rays_per_step": rays // args.steps, "kernel_ms_per_step": kernel_ms},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batch,
+        "served": served,
     }
     print(json.dumps(line))
 

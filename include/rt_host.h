@@ -104,6 +104,53 @@ typedef struct rth_render_stats {
 int rth_render(rth_scene* s, int32_t device, int32_t precision, int32_t progressive, int32_t max_pass, double* sum,
                uint32_t* count, uint8_t* rgb, rth_render_stats* stats);
 
+/* ---------------------------------------------------------------------- */
+/* The served render loop: render requests over scenes resident on one GPU,
+ * scheduled in ticks of batched passes (rtg_render_batch). Previews render
+ * all samples of their pass in the next tick; finals advance their
+ * progressive passes (1, 1, 2, 4, ... capped at max_pass, renderer.cpp:66-69)
+ * by at most tick_paths paths per tick; requests are admitted first come
+ * first served while their accumulation buffers fit budget_bytes. It has no
+ * counterpart in the reference, whose editor backend starts one `rt` process
+ * per job (web/backend/jobs.go:130-259). Errors: 1 + rth_last_error(). */
+typedef struct rts_server rts_server;
+
+typedef struct rts_request {
+  int32_t scene; /* index returned by rts_add_scene */
+  int32_t width, height, spp, max_depth, max_pass;
+  int32_t final_kind; /* 0 preview, 1 final */
+  int32_t pad;
+  uint64_t seed;
+  double arrival_ms; /* from the start of the run; non-decreasing */
+} rts_request;
+
+typedef struct rts_result {
+  double arrival_ms, start_ms, first_frame_ms, done_ms; /* from the start of the run */
+  uint64_t rays, shadow_rays;
+  uint32_t frames, spp_done;
+  uint64_t image_hash; /* FNV-1a over 64-bit words: every final sum (double
+                          bits), then every count widened to 64 bits */
+} rts_result;
+
+typedef struct rts_stats {
+  double wall_ms;
+  uint64_t rays; /* closest-hit + shadow rays */
+  uint32_t ticks, max_active;
+  uint64_t peak_accum_bytes;
+} rts_stats;
+
+/* budget_bytes 0: 64 GiB; tick_paths 0: 8 Mi paths. */
+int rts_create(int32_t device, uint64_t budget_bytes, uint32_t tick_paths, rts_server** out);
+/* Uploads a compiled scene (kept resident); its render settings supply the
+ * tone map, exposure, roulette depth and light sampling of every request. */
+int rts_add_scene(rts_server* s, const rth_scene* scene, int32_t* index);
+/* Replays n requests in real time: served = 1 for the tick scheduler, 0 for
+ * the plain loop (one request at a time through rtg_render_pass). Blocks
+ * until every request is done. */
+int rts_run(rts_server* s, const rts_request* reqs, uint32_t n, int32_t served, rts_result* results,
+            rts_stats* stats);
+void rts_free(rts_server* s);
+
 #ifdef __cplusplus
 }
 #endif

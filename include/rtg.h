@@ -177,6 +177,12 @@ rtg_status rtg_device_info(rtg_context* ctx, char* name, size_t name_len, int* m
  * Replaces the scene argument of Renderer (rendering/renderer.hpp:52). */
 rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* desc, rtg_scene** out);
 rtg_status rtg_scene_release(rtg_scene* scene);
+/* A handle on `base`'s device arrays with another camera — the same scene
+ * rendered at another image size needs the camera set up for that size
+ * (core/camera.hpp:29-51). No geometry is copied; the arrays are freed when
+ * the last handle sharing them is released. */
+rtg_status rtg_scene_with_camera(rtg_context* ctx, const rtg_scene* base, const rtg_camera* camera,
+                                 rtg_scene** out);
 /* Device bytes held by the scene (admission by HBM footprint). */
 uint64_t rtg_scene_bytes(const rtg_scene* scene);
 

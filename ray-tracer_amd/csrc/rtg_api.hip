@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -50,6 +51,8 @@ struct rtg_context {
   uint32_t* batch_plans = nullptr;  // 4 x (kMaxBatchJobs + 1): gen, acc, act, shd
   void* batch_host = nullptr;       // job table + gen/acc plans, pinned
   uint32_t batch_occ[3][3] = {};    // per stack class: trace, shade, other workgroups per CU
+  uint8_t* rgb_dev = nullptr;       // rtg_resolve staging
+  size_t rgb_cap = 0;
   // Profiling: an event pair per launch, resolved when the call completes.
   bool profiling = false;
   std::vector<hipEvent_t> events;
@@ -59,10 +62,22 @@ struct rtg_context {
   LaunchHooks hooks{};
 };
 
+// Device arrays of a scene, shared by the handles made from it with other
+// cameras (rtg_scene_with_camera); freed with the last of them.
+struct DeviceArrays {
+  int device = 0;
+  std::vector<void*> ptrs;
+  ~DeviceArrays() {
+    if (ptrs.empty()) return;
+    hipSetDevice(device);
+    for (void* p : ptrs) hipFree(p);
+  }
+};
+
 struct rtg_scene {
   rtg_context* ctx = nullptr;
   DevScene dev{};
-  std::vector<void*> allocs;
+  std::shared_ptr<DeviceArrays> arrays;
   uint64_t bytes = 0;
   uint32_t stack = 0;  // BVH depth + 1
 };
@@ -115,6 +130,16 @@ float bits_f(uint32_t u) {
 }
 float4 f4(float x, float y, float z, float w) { return float4{x, y, z, w}; }
 
+// The camera as generate() reads it: FP64 for the exact mode, FP32 origin,
+// corner - origin, du, dv, u, v and lens radius for the fast mode.
+void set_camera(DevScene& S, const rtg_camera& c) {
+  S.cam = c;
+  const double cam[19] = {c.origin.x, c.origin.y, c.origin.z, c.corner.x - c.origin.x, c.corner.y - c.origin.y,
+                          c.corner.z - c.origin.z, c.du.x, c.du.y, c.du.z, c.dv.x, c.dv.y, c.dv.z,
+                          c.u.x, c.u.y, c.u.z, c.v.x, c.v.y, c.v.z, c.lens_radius};
+  for (int i = 0; i < 19; ++i) S.cam32[i] = (float)cam[i];
+}
+
 struct Upload {
   rtg_scene* s;
   rtg_context* ctx;
@@ -126,7 +151,7 @@ struct Upload {
     }
     void* p = nullptr;
     HIP_TRY(ctx, hipMalloc(&p, n * sizeof(T)));
-    s->allocs.push_back(p);
+    s->arrays->ptrs.push_back(p);
     s->bytes += n * sizeof(T);
     HIP_TRY(ctx, hipMemcpy(p, host, n * sizeof(T), hipMemcpyHostToDevice));
     *dev = static_cast<const T*>(p);
@@ -260,6 +285,7 @@ rtg_status rtg_close(rtg_context* ctx) {
   if (ctx->batch_jobs) hipFree(ctx->batch_jobs);
   if (ctx->batch_plans) hipFree(ctx->batch_plans);
   if (ctx->batch_host) hipHostFree(ctx->batch_host);
+  if (ctx->rgb_dev) hipFree(ctx->rgb_dev);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
   for (hipEvent_t e : ctx->events) hipEventDestroy(e);
@@ -294,10 +320,11 @@ rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* d, rtg_scene
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   auto* s = new rtg_scene;
   s->ctx = ctx;
+  s->arrays = std::make_shared<DeviceArrays>();
+  s->arrays->device = ctx->device;
   Upload up{s, ctx};
   DevScene& S = s->dev;
   auto cleanup = [&](rtg_status st) {
-    for (void* p : s->allocs) hipFree(p);
     delete s;
     return st;
   };
@@ -474,13 +501,8 @@ rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* d, rtg_scene
     S.root_lo[0] = round_down(r.lo.x); S.root_lo[1] = round_down(r.lo.y); S.root_lo[2] = round_down(r.lo.z);
     S.root_hi[0] = round_up(r.hi.x); S.root_hi[1] = round_up(r.hi.y); S.root_hi[2] = round_up(r.hi.z);
   }
-  S.cam = d->camera;
+  set_camera(S, d->camera);
   S.bg = d->background;
-  const rtg_camera& c = d->camera;
-  const double cam[19] = {c.origin.x, c.origin.y, c.origin.z, c.corner.x - c.origin.x, c.corner.y - c.origin.y,
-                          c.corner.z - c.origin.z, c.du.x, c.du.y, c.du.z, c.dv.x, c.dv.y, c.dv.z,
-                          c.u.x, c.u.y, c.u.z, c.v.x, c.v.y, c.v.z, c.lens_radius};
-  for (int i = 0; i < 19; ++i) S.cam32[i] = (float)cam[i];
   const rtg_background& b = d->background;
   const double bg[7] = {b.top.x, b.top.y, b.top.z, b.bottom.x, b.bottom.y, b.bottom.z, b.intensity};
   for (int i = 0; i < 7; ++i) S.bg32[i] = (float)bg[i];
@@ -489,11 +511,21 @@ rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* d, rtg_scene
   return RTG_OK;
 }
 
+rtg_status rtg_scene_with_camera(rtg_context* ctx, const rtg_scene* base, const rtg_camera* camera, rtg_scene** out) {
+  if (!ctx || !base || !camera || !out) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_scene_with_camera: NULL argument");
+  if (base->ctx->device != ctx->device)
+    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_scene_with_camera: the scene lives on another device");
+  auto* s = new rtg_scene(*base);
+  s->ctx = ctx;
+  set_camera(s->dev, *camera);
+  *out = s;
+  return RTG_OK;
+}
+
 rtg_status rtg_scene_release(rtg_scene* s) {
   if (!s) return RTG_OK;
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
-  for (void* p : s->allocs) hipFree(p);
   delete s;
   return RTG_OK;
 }
@@ -564,12 +596,17 @@ rtg_status rtg_resolve(rtg_context* ctx, const rtg_accum* a, int32_t tone_map, d
   if (!ctx || !a || !rgb || tone_map < 0 || tone_map > 2)
     return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_resolve: bad arguments");
   const size_t n = (size_t)a->w * a->h * 3;
-  uint8_t* dev = nullptr;
-  HIP_TRY(ctx, hipMallocAsync((void**)&dev, n, ctx->stream));
-  run_resolve(a->sum, a->count, a->w, a->h, tone_map, exposure, dev, ctx->stream);
+  if (ctx->rgb_cap < n) {  // device staging of the 8-bit image, kept per context
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->rgb_dev) HIP_TRY(ctx, hipFree(ctx->rgb_dev));
+    ctx->rgb_dev = nullptr;
+    ctx->rgb_cap = 0;
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->rgb_dev, n));
+    ctx->rgb_cap = n;
+  }
+  run_resolve(a->sum, a->count, a->w, a->h, tone_map, exposure, ctx->rgb_dev, ctx->stream);
   HIP_TRY(ctx, hipGetLastError());
-  HIP_TRY(ctx, hipMemcpyAsync(rgb, dev, n, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(ctx, hipFreeAsync(dev, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(rgb, ctx->rgb_dev, n, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return RTG_OK;
 }

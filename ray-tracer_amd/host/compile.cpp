@@ -140,6 +140,7 @@ rtg_camera setup_camera(const CameraDesc& d, int width, int height) {
 CompiledScene compile_scene(const SceneDesc& desc) {
   CompiledScene s;
   s.settings = desc.render;
+  s.camera_desc = desc.camera;
   s.camera = setup_camera(desc.camera, desc.render.width, desc.render.height);
   s.background.kind = desc.background.solid ? RTG_BG_SOLID : RTG_BG_GRADIENT;
   s.background.top = to_rtg(d3(desc.background.top));

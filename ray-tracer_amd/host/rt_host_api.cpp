@@ -6,6 +6,7 @@
 
 #include "renderer.hpp"
 #include "rt_host.h"
+#include "server.hpp"
 
 using namespace rtm;
 
@@ -202,5 +203,70 @@ int rth_render(rth_scene* s, int32_t device, int32_t precision, int32_t progress
     return 1;
   }
 }
+
+// ---------------------------------------------------------------------------
+struct rts_server {
+  Server server;
+  rts_server(int device, uint64_t budget, uint32_t tick) : server(device, budget, tick) {}
+};
+
+int rts_create(int32_t device, uint64_t budget_bytes, uint32_t tick_paths, rts_server** out) {
+  if (!out) {
+    g_last_error = "rts_create: out is NULL";
+    return 1;
+  }
+  try {
+    *out = new rts_server(device, budget_bytes, tick_paths);
+    return 0;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return 1;
+  }
+}
+
+int rts_add_scene(rts_server* s, const rth_scene* scene, int32_t* index) {
+  if (!s || !scene) {
+    g_last_error = "rts_add_scene: NULL argument";
+    return 1;
+  }
+  try {
+    const int i = s->server.add_scene(scene->compiled);
+    if (index) *index = i;
+    return 0;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return 1;
+  }
+}
+
+int rts_run(rts_server* s, const rts_request* reqs, uint32_t n, int32_t served, rts_result* results,
+            rts_stats* stats) {
+  if (!s || (n && (!reqs || !results))) {
+    g_last_error = "rts_run: NULL argument";
+    return 1;
+  }
+  try {
+    std::vector<ServeRequest> rq(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      const rts_request& q = reqs[i];
+      rq[i] = ServeRequest{q.scene, q.width, q.height, q.spp, q.max_depth, q.max_pass, q.final_kind, q.seed,
+                           q.arrival_ms};
+    }
+    std::vector<ServeResult> res;
+    const ServeStats st = s->server.run(rq, served != 0, res);
+    for (uint32_t i = 0; i < n; ++i) {
+      const ServeResult& r = res[i];
+      results[i] = rts_result{r.arrival_ms, r.start_ms, r.first_frame_ms, r.done_ms, r.rays,
+                              r.shadow_rays, r.frames,   r.spp_done,       r.image_hash};
+    }
+    if (stats) *stats = rts_stats{st.wall_ms, st.rays, st.ticks, st.max_active, st.peak_accum_bytes};
+    return 0;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return 1;
+  }
+}
+
+void rts_free(rts_server* s) { delete s; }
 
 }  // extern "C"

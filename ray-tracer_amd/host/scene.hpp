@@ -127,6 +127,7 @@ struct SceneInfo {  // core/scene.hpp:25-28
 // Render-ready scene: exactly the arrays rtg_scene_create() consumes.
 struct CompiledScene {
   RenderSettings settings;
+  CameraDesc camera_desc;  // to set the camera up for another image size
   rtg_camera camera{};
   rtg_background background{};
   std::vector<rtg_material> materials;

@@ -136,6 +136,7 @@ def load_rtg():
         "rtg_device_info": (C.c_int32, [_VP, C.c_char_p, C.c_size_t, P(C.c_int), P(C.c_uint64)]),
         "rtg_scene_create": (C.c_int32, [_VP, P(SceneDesc), P(_VP)]),
         "rtg_scene_release": (C.c_int32, [_VP]),
+        "rtg_scene_with_camera": (C.c_int32, [_VP, _VP, P(Camera), P(_VP)]),
         "rtg_scene_bytes": (C.c_uint64, [_VP]),
         "rtg_accum_create": (C.c_int32, [_VP, C.c_int32, C.c_int32, P(_VP)]),
         "rtg_accum_release": (C.c_int32, [_VP]),
@@ -163,7 +164,7 @@ def load_rtg():
 def rtg_symbols():
     """Names declared in include/rtg.h (checked by the CPU tests)."""
     return ["rtg_open", "rtg_close", "rtg_last_error", "rtg_abi_version", "rtg_device_info", "rtg_scene_create",
-            "rtg_scene_release", "rtg_scene_bytes", "rtg_accum_create", "rtg_accum_release", "rtg_accum_clear",
+            "rtg_scene_release", "rtg_scene_with_camera", "rtg_scene_bytes", "rtg_accum_create", "rtg_accum_release", "rtg_accum_clear",
             "rtg_accum_read", "rtg_accum_min_samples", "rtg_render_pass", "rtg_render_batch", "rtg_resolve",
             "rtg_buffer_create",
             "rtg_buffer_release", "rtg_buffer_write", "rtg_buffer_read", "rtg_trace", "rtg_trace_host",
@@ -340,6 +341,22 @@ class HostRenderStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+REQUEST_DTYPE = np.dtype([("scene", "<i4"), ("width", "<i4"), ("height", "<i4"), ("spp", "<i4"),
+                          ("max_depth", "<i4"), ("max_pass", "<i4"), ("final_kind", "<i4"), ("pad", "<i4"),
+                          ("seed", "<u8"), ("arrival_ms", "<f8")])
+RESULT_DTYPE = np.dtype([("arrival_ms", "<f8"), ("start_ms", "<f8"), ("first_frame_ms", "<f8"), ("done_ms", "<f8"),
+                         ("rays", "<u8"), ("shadow_rays", "<u8"), ("frames", "<u4"), ("spp_done", "<u4"),
+                         ("image_hash", "<u8")])
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall_ms", C.c_double), ("rays", C.c_uint64), ("ticks", C.c_uint32), ("max_active", C.c_uint32),
+                ("peak_accum_bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _rth = None
 
 
@@ -371,6 +388,10 @@ def load_host():
         "rth_dump_scene": (C.c_int64, [C.c_char_p, C.c_char_p, C.c_uint64]),
         "rth_render": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _VP, _VP, _VP,
                                  P(HostRenderStats)]),
+        "rts_create": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint32, P(_VP)]),
+        "rts_add_scene": (C.c_int, [_VP, _VP, P(C.c_int32)]),
+        "rts_run": (C.c_int, [_VP, _VP, C.c_uint32, C.c_int32, _VP, P(ServeStats)]),
+        "rts_free": (None, [_VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -382,7 +403,46 @@ def load_host():
 def host_symbols():
     return ["rt_validate", "rt_message", "rt_threads", "rt_running", "rt_cancel", "rt_start", "rt_wait",
             "rth_scene_load", "rth_scene_free", "rth_last_error", "rth_scene_desc", "rth_scene_settings",
-            "rth_scene_info", "rth_scene_bvh_build_ms", "rth_dump_scene", "rth_render"]
+            "rth_scene_info", "rth_scene_bvh_build_ms", "rth_dump_scene", "rth_render", "rts_create",
+            "rts_add_scene", "rts_run", "rts_free"]
+
+
+class Server:
+    """The served render loop (rts_*): scenes resident on one GPU, requests
+    replayed in real time by the tick scheduler or by the plain loop."""
+
+    def __init__(self, device=0, budget_bytes=0, tick_paths=0):
+        self.lib = load_host()
+        h = C.c_void_p()
+        if self.lib.rts_create(device, budget_bytes, tick_paths, C.byref(h)):
+            raise RtgError(self.lib.rth_last_error().decode())
+        self.h = h
+        self._scenes = []
+
+    def add_scene(self, host_scene):
+        i = C.c_int32()
+        if self.lib.rts_add_scene(self.h, host_scene.h, C.byref(i)):
+            raise RtgError(self.lib.rth_last_error().decode())
+        self._scenes.append(host_scene)
+        return i.value
+
+    def run(self, requests, served=True):
+        """requests: structured array of REQUEST_DTYPE sorted by arrival.
+        Returns (results array of RESULT_DTYPE, stats dict)."""
+        reqs = np.ascontiguousarray(requests, dtype=REQUEST_DTYPE)
+        res = np.zeros(len(reqs), RESULT_DTYPE)
+        st = ServeStats()
+        if self.lib.rts_run(self.h, _ptr(reqs), len(reqs), 1 if served else 0, _ptr(res), C.byref(st)):
+            raise RtgError(self.lib.rth_last_error().decode())
+        return res, st.as_dict()
+
+    def close(self):
+        if self.h:
+            self.lib.rts_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 FRAME_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.POINTER(C.c_uint8), C.c_void_p)

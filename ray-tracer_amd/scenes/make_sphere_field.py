@@ -38,7 +38,8 @@ class Pcg32:
         return lo + (hi - lo) * (self.next_u32() / 4294967296.0)
 
 
-def build(seed=3):
+def build(seed=3, half=11):
+    """The scene as a dict; `half` sets the grid, a, b in [-half, half)."""
     rng = Pcg32(seed)
     mats = {
         "ground": {"type": "lambertian", "albedo": [0.5, 0.5, 0.5], "albedo2": [0.32, 0.36, 0.3],
@@ -51,8 +52,8 @@ def build(seed=3):
     objs = [{"type": "plane", "name": "ground", "point": [0, 0, 0], "normal": [0, 1, 0], "material": "ground"}]
     features = [(0.0, 0.0), (-4.0, 0.0), (4.0, 0.0)]
     k = 0
-    for a in range(-11, 11):
-        for b in range(-11, 11):
+    for a in range(-half, half):
+        for b in range(-half, half):
             choose = rng.uniform()
             x = a + 0.9 * rng.uniform()
             z = b + 0.9 * rng.uniform()

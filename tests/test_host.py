@@ -25,7 +25,7 @@ RT = os.path.join(ROOT, "ray-tracer_amd", "bin", "rt")
 def header_functions(name):
     text = open(os.path.join(ROOT, "include", name)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b((?:rtg|rth|rt)_[a-z0-9_]+)\s*\(", text)) - {"rt_frame_fn", "rt_done_fn"})
+    return sorted(set(re.findall(r"\b((?:rtg|rth|rts|rt)_[a-z0-9_]+)\s*\(", text)) - {"rt_frame_fn", "rt_done_fn"})
 
 
 @pytest.mark.parametrize("lib,header", [("librtg.so", "rtg.h"), ("librthost.so", "rt_host.h")])
