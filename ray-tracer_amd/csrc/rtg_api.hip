@@ -855,7 +855,7 @@ rtg_status run_group(rtg_context* ctx, const rtg_job* jobs, const Segment* seg, 
     hgen[i] = gen_blocks;
     gen_blocks += (generate_threads(P) + kBlock - 1) / kBlock;
     hacc[i] = acc_blocks;
-    acc_blocks += (P.n_pix + kBlock - 1) / kBlock;
+    acc_blocks += accumulate_blocks<float>(P);
     path_blocks += (P.n_paths + kBlock - 1) / kBlock;
     depth = std::max(depth, P.max_depth);
     stack = std::max(stack, J.scene->stack);

@@ -24,7 +24,7 @@ void chunk_impl(const DevScene& S, const PathBuffers<float>& B, const ChunkParam
       if (nee) RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
     }
   }
-  RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_accumulate<float>, dim3(grid_of(P.n_pix)), dim3(kBlock), 0, s, B, P);
+  RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_accumulate<float>, dim3(accumulate_blocks<float>(P)), dim3(kBlock), 0, s, B, P);
 }
 
 template <int STACK>

@@ -58,7 +58,7 @@ void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParam
       if (nee) RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
     }
   }
-  RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_accumulate<double>, dim3(grid_of(P.n_pix)), dim3(kBlock), 0, s, B, P);
+  RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_accumulate<double>, dim3(accumulate_blocks<double>(P)), dim3(kBlock), 0, s, B, P);
 }
 
 void run_trace64(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,

@@ -468,36 +468,86 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, PathBuffers<T> B,
 
 // accumulate: the invalid-sample guard and Framebuffer::add in sample order
 // (renderer.cpp:39-45, framebuffer.hpp:31-35).
+//
+// With pixel-major path ids a pixel's k samples are contiguous, so one
+// thread per pixel would read with a stride of k values. Instead a block
+// takes a run of whole pixels (at most kAccTile<T> samples), loads it into
+// LDS with coalesced reads (one pad slot per pixel keeps the per-pixel reads
+// free of bank conflicts) and each thread adds one pixel's samples in
+// sample order from LDS. Passes wider than the tile, and sample-major ids,
+// read global memory directly.
 template <class T>
-__device__ __forceinline__ void accumulate_body(const PathBuffers<T>& B, const ChunkParams& P, uint32_t lb) {
-  const uint32_t j = lb * kBlock + threadIdx.x;
-  uint32_t st[kStCount] = {0};
-  if (j < P.n_pix) {
-    const uint32_t pixel = P.p0 + j;
-    double sx = P.acc_sum[3 * (size_t)pixel], sy = P.acc_sum[3 * (size_t)pixel + 1],
-           sz = P.acc_sum[3 * (size_t)pixel + 2];
-    for (uint32_t s = 0; s < P.k; ++s) {
-      const uint32_t pid = pid_of(P, j, s);
-      T x = B.lx[pid], y = B.ly[pid], z = B.lz[pid];
-      if (!(r_isfinite(x) && r_isfinite(y) && r_isfinite(z)) || x < 0 || y < 0 || z < 0) {
-        ++st[kStInvalid];
-        x = y = z = 0;
-      }
-      sx += (double)x;
-      sy += (double)y;
-      sz += (double)z;
+constexpr uint32_t kAccTile = 4096 / sizeof(T);
+template <class T>
+constexpr uint32_t kAccLds = 3 * 2 * kAccTile<T>;  // values of LDS per block (x, y, z with pads)
+
+template <class T>
+__host__ __device__ inline uint32_t acc_pixels_per_block(const ChunkParams& P) {
+  return (P.sample_major || P.k > kAccTile<T>) ? (uint32_t)kBlock : kAccTile<T> / P.k;
+}
+template <class T>
+__host__ __device__ inline uint32_t accumulate_blocks(const ChunkParams& P) {
+  const uint32_t ppb = acc_pixels_per_block<T>(P);
+  return (P.n_pix + ppb - 1) / ppb;
+}
+
+template <class T>
+__device__ __forceinline__ void add_pixel(const ChunkParams& P, uint32_t pixel, const T* x, const T* y, const T* z,
+                                          uint32_t stride, uint32_t* st) {
+  double sx = P.acc_sum[3 * (size_t)pixel], sy = P.acc_sum[3 * (size_t)pixel + 1],
+         sz = P.acc_sum[3 * (size_t)pixel + 2];
+  for (uint32_t s = 0; s < P.k; ++s) {
+    T a = x[s * stride], b = y[s * stride], c = z[s * stride];
+    if (!(r_isfinite(a) && r_isfinite(b) && r_isfinite(c)) || a < 0 || b < 0 || c < 0) {
+      ++st[kStInvalid];
+      a = b = c = 0;
     }
-    P.acc_sum[3 * (size_t)pixel] = sx;
-    P.acc_sum[3 * (size_t)pixel + 1] = sy;
-    P.acc_sum[3 * (size_t)pixel + 2] = sz;
-    P.acc_count[pixel] += P.k;
-    st[kStCamera] = P.k;
+    sx += (double)a;
+    sy += (double)b;
+    sz += (double)c;
+  }
+  P.acc_sum[3 * (size_t)pixel] = sx;
+  P.acc_sum[3 * (size_t)pixel + 1] = sy;
+  P.acc_sum[3 * (size_t)pixel + 2] = sz;
+  P.acc_count[pixel] += P.k;
+  st[kStCamera] += P.k;
+}
+
+template <class T>
+__device__ __forceinline__ void accumulate_body(const PathBuffers<T>& B, const ChunkParams& P, uint32_t lb, T* tile) {
+  uint32_t st[kStCount] = {0};
+  if (P.sample_major || P.k > kAccTile<T>) {
+    const uint32_t j = lb * kBlock + threadIdx.x;
+    if (j < P.n_pix) {
+      const uint32_t p0 = pid_of(P, j, 0), stride = P.sample_major ? P.n_pix : 1u;
+      add_pixel<T>(P, P.p0 + j, B.lx + p0, B.ly + p0, B.lz + p0, stride, st);
+    }
+  } else {
+    const uint32_t ppb = kAccTile<T> / P.k, j0 = lb * ppb;
+    const uint32_t np = P.n_pix - j0 < ppb ? P.n_pix - j0 : ppb;
+    const uint32_t n = np * P.k, base = P.pid_base + j0 * P.k;
+    T* tx = tile;
+    T* ty = tile + 2 * kAccTile<T>;
+    T* tz = tile + 4 * kAccTile<T>;
+    for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
+      const uint32_t at = e + e / P.k;  // pixel e / k at (k + 1) * (e / k)
+      tx[at] = B.lx[base + e];
+      ty[at] = B.ly[base + e];
+      tz[at] = B.lz[base + e];
+    }
+    __syncthreads();
+    for (uint32_t jj = threadIdx.x; jj < np; jj += kBlock) {
+      const uint32_t at = jj * (P.k + 1);
+      add_pixel<T>(P, P.p0 + j0 + jj, tx + at, ty + at, tz + at, 1u, st);
+    }
+    __syncthreads();  // the tile is reused by the block's next job
   }
   flush_stats(P, st, lb);
 }
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_accumulate(PathBuffers<T> B, ChunkParams P) {
-  accumulate_body<T>(B, P, blockIdx.x);
+  __shared__ T tile[kAccLds<T>];
+  accumulate_body<T>(B, P, blockIdx.x, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -850,7 +900,8 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_batch_accumulate(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
                                                              BatchPlans L) {
   __shared__ uint32_t s_plan[kMaxBatchJobs + 1];
-  batch_loop(L.acc, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) { accumulate_body<T>(B, jobs[j].P, lb); });
+  __shared__ T tile[kAccLds<T>];
+  batch_loop(L.acc, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) { accumulate_body<T>(B, jobs[j].P, lb, tile); });
 }
 
 }  // namespace rtg

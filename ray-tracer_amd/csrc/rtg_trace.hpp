@@ -240,6 +240,7 @@ __device__ inline bool any64(const DevScene& S, bool use_bvh, V3<double> o, V3<d
 
 struct Ray32 {
   float ox, oy, oz, dx, dy, dz, ix, iy, iz;
+  float qx, qy, qz;  // o / d: slab distances are one fma each
 };
 
 __device__ __forceinline__ float safe_inv(float d) {
@@ -254,21 +255,25 @@ __device__ __forceinline__ Ray32 make_ray32(float ox, float oy, float oz, float 
   r.ox = ox; r.oy = oy; r.oz = oz;
   r.dx = dx; r.dy = dy; r.dz = dz;
   r.ix = safe_inv(dx); r.iy = safe_inv(dy); r.iz = safe_inv(dz);
+  r.qx = ox * r.ix; r.qy = oy * r.iy; r.qz = oz * r.iz;
   return r;
 }
 
-// Slab test of one float box; returns the entry distance in tn.
+// Slab test of one float box; returns the entry distance in tn. Plane
+// distances are fma(bound, 1/d, -o/d); the far distance is widened by
+// 5e-7 relative so that the rounding of o/d never culls a box the ray
+// touches.
 __device__ __forceinline__ bool box32(const Ray32& r, float lx, float ly, float lz, float hx, float hy, float hz,
                                       float tmin, float tmax, float& tn) {
-  const float t0x = (lx - r.ox) * r.ix, t1x = (hx - r.ox) * r.ix;
-  const float t0y = (ly - r.oy) * r.iy, t1y = (hy - r.oy) * r.iy;
-  const float t0z = (lz - r.oz) * r.iz, t1z = (hz - r.oz) * r.iz;
+  const float t0x = __builtin_fmaf(lx, r.ix, -r.qx), t1x = __builtin_fmaf(hx, r.ix, -r.qx);
+  const float t0y = __builtin_fmaf(ly, r.iy, -r.qy), t1y = __builtin_fmaf(hy, r.iy, -r.qy);
+  const float t0z = __builtin_fmaf(lz, r.iz, -r.qz), t1z = __builtin_fmaf(hz, r.iz, -r.qz);
   const float nx = r_fmin(t0x, t1x), fx = r_fmax(t0x, t1x);
   const float ny = r_fmin(t0y, t1y), fy = r_fmax(t0y, t1y);
   const float nz = r_fmin(t0z, t1z), fz = r_fmax(t0z, t1z);
   tn = r_fmax(r_fmax(nx, ny), r_fmax(nz, tmin));
-  const float tf = r_fmin(r_fmin(fx, fy), r_fmin(fz, tmax));
-  return tn <= tf;
+  const float tf = r_fmin(r_fmin(fx, fy), fz) * 1.0000005f;
+  return tn <= r_fmin(tf, tmax);
 }
 
 // Slot test (triangle: Möller–Trumbore on precomputed edges; sphere: the
