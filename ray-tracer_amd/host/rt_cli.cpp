@@ -8,7 +8,8 @@
 // 2 scene error, 3 write error, 130 cancelled) are those of the reference
 // CLI (apps/rt_cli.cpp:6-10, 32-209). Added: --precision f32|f64 and
 // --device N; DONE also reports grays_per_sec. --threads and --tile are
-// accepted for compatibility (the GPU schedules its own work).
+// accepted for compatibility (the GPU schedules its own work). --serve runs
+// many jobs in one process (serve_stream.cpp).
 #include <atomic>
 #include <csignal>
 #include <cstdio>
@@ -19,6 +20,10 @@
 #include <string>
 
 #include "renderer.hpp"
+
+namespace rtm {
+int serve_main(int device, bool supersede, uint32_t tick_paths, uint64_t cache_bytes, bool quiet);
+}
 
 using namespace rtm;
 
@@ -49,7 +54,13 @@ void usage() {
                "  --snapshot-dir DIR     also save each pass as DIR/spp_NNNNN.png\n"
                "  --validate             only parse + validate the scene\n"
                "  --dump-scene           print the normalised scene JSON and exit\n"
-               "  -q, --quiet            no progress output\n";
+               "  -q, --quiet            no progress output\n"
+               "  --serve                serve jobs given as JSON lines on stdin (no scene argument):\n"
+               "                         {\"job\":N,\"scene\":{...},overrides...} or {\"cancel\":N}; streams\n"
+               "                         FRAME/DONE/ERROR lines tagged with the job id\n"
+               "  --supersede            (serve) a new job cancels the running ones\n"
+               "  --tick-paths N         (serve) paths of large jobs per batched pass, default 8388608\n"
+               "  --cache-mb N           (serve) resident compiled scenes, default 8192\n";
 }
 
 std::string json_str(const std::string& s) {
@@ -72,9 +83,9 @@ int main(int argc, char** argv) {
   std::optional<double> exposure;
   std::optional<std::string> integrator;
   bool no_bvh = false, no_nee = false, single_pass = false, stream = false, quiet = false;
-  bool validate_only = false, dump = false, save_every = false;
+  bool validate_only = false, dump = false, save_every = false, serve = false, supersede = false;
   int32_t precision = RTG_PRECISION_F32;
-  int device = 0;
+  int device = 0, tick_paths = 0, cache_mb = 0;
 
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
@@ -115,6 +126,10 @@ int main(int argc, char** argv) {
     else if (a == "--dump-scene") dump = true;
     else if (a == "-q" || a == "--quiet") quiet = true;
     else if (a == "--device") device = next_int();
+    else if (a == "--serve") serve = true;
+    else if (a == "--supersede") supersede = true;
+    else if (a == "--tick-paths") tick_paths = next_int();
+    else if (a == "--cache-mb") cache_mb = next_int();
     else if (a == "--precision") {
       const std::string p = next();
       if (p == "f32") precision = RTG_PRECISION_F32;
@@ -134,6 +149,8 @@ int main(int argc, char** argv) {
       scene_path = a;
     }
   }
+  if (serve) return rtm::serve_main(device, supersede, (uint32_t)std::max(0, tick_paths),
+                                    (uint64_t)std::max(0, cache_mb) << 20, quiet);
   if (scene_path.empty()) {
     usage();
     return 1;
