@@ -10,10 +10,10 @@
 //   * node_visits counts node boxes tested, prim_tests primitive tests.
 // FP64 functions are the reference algorithm on the reference layout and
 // reproduce its hits bit for bit. FP32 functions run on the two-child
-// layout: both child boxes are tested from one 64-byte record, the far
-// child is pushed with its entry distance and skipped on pop if a closer
-// hit has arrived since (the reference's pop-time box test, without
-// reloading the box).
+// layout: both child boxes are tested from one 64-byte record against the
+// current closest t, and only the far child's reference is pushed; a popped
+// child is entered without a second box test (its box was already tested,
+// and any primitive in it still has to beat the closest t).
 #pragma once
 #include "rtg_math.hpp"
 #include "rtg_scene.hpp"
