@@ -241,7 +241,8 @@ struct DrawSrc<float> {
   __device__ static void store(const PathBuffers<float>&, uint32_t, const HashDraws&) {}
 };
 
-__device__ __forceinline__ LdsStack lds_stack(uint32_t* ref) { return LdsStack{ref + threadIdx.x, (uint32_t)blockDim.x}; }
+static_assert(kBlock == (int)kStackStride, "traversal stacks are laid out for kBlock threads");
+__device__ __forceinline__ LdsStack lds_stack(uint32_t* ref) { return LdsStack{ref + threadIdx.x}; }
 
 // ---------------------------------------------------------------------------
 // Path id -> (pixel, sample). Pixel-major keeps the samples of one pixel

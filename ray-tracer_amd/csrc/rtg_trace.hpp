@@ -352,16 +352,59 @@ struct Hit32 {
   int plane;
 };
 
-// LDS stack: entry k of this lane lives at [k * stride].
+// LDS stack: entry k of this lane lives at ref[k * kStackStride] (one
+// column per thread of a traversal workgroup, which has kStackStride threads).
 // The FP32 stack holds child refs only (4 bytes per entry): half the LDS of
 // a {ref, entry distance} stack doubles the resident waves, which is worth
 // more than culling far entries on pop (measured on the torus workload:
 // closest-hit 40.4 -> 29.0 ms per render for 5 % more node visits). A far
 // child popped after a closer hit is visited again; its boxes lie inside the
 // pushed box, so it cannot change the result.
+constexpr uint32_t kStackStride = 128;
 struct LdsStack {
   uint32_t* ref;
-  uint32_t stride;
+};
+
+// "No node": never a valid child reference (a leaf's first slot plus its
+// count never exceeds 2^25), and it has the leaf bit, so inner() is false.
+constexpr uint32_t kNoRef = 0xFFFFFFFFu;
+
+// Traversal state of one ray, walked as two loops per round (Aila & Laine,
+// "Understanding the Efficiency of Ray Traversal on GPUs", 2009): descend
+// inner nodes until every lane of the wave has reached a leaf, parking the
+// first leaf a lane meets and continuing from its stack, then test the
+// parked leaves. Lanes that reach a leaf early keep descending instead of
+// idling while the rest of the wave finishes its inner nodes.
+struct Walk {
+  uint32_t ref;   // next node to visit (kNoRef: traversal done)
+  uint32_t leaf;  // parked leaf (kNoRef: none)
+  uint32_t sp;    // stack depth
+
+  __device__ __forceinline__ bool inner() const { return !(ref & kLeafBit); }
+  __device__ __forceinline__ uint32_t pop(LdsStack s) {
+    if (sp == 0) return kNoRef;
+    --sp;
+    return s.ref[sp * kStackStride];
+  }
+  // Parks ref if it is a leaf and no leaf is parked, continuing from the stack.
+  __device__ __forceinline__ void park(LdsStack s) {
+    if (leaf == kNoRef && ref != kNoRef && (ref & kLeafBit)) {
+      leaf = ref;
+      ref = pop(s);
+    }
+  }
+  // One inner node with children `first` (visited first) and `second` and
+  // whether their boxes were hit. The push slot is written unconditionally
+  // and kept only when both are hit: at an inner node of depth d the stack
+  // holds at most d entries, so the slot lies below the BVH depth + 1.
+  __device__ __forceinline__ void step(LdsStack s, uint32_t first, uint32_t second, bool hf, bool hs) {
+    s.ref[sp * kStackStride] = second;
+    sp += (hf && hs) ? 1u : 0u;
+    ref = hf ? first : hs ? second : pop(s);
+    park(s);
+  }
+  // Every lane still descending has a leaf parked (a wave-wide vote).
+  __device__ __forceinline__ bool all_parked() const { return __builtin_amdgcn_ballot_w64(leaf == kNoRef) == 0; }
 };
 
 __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& r, float tmin, float tmax,
@@ -390,35 +433,26 @@ __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& 
   ++c.visits;
   if (!box32(r, S.root_lo[0], S.root_lo[1], S.root_lo[2], S.root_hi[0], S.root_hi[1], S.root_hi[2], tmin, h.t, tn))
     return h;
-  const bool neg[3] = {r.dx < 0.0f, r.dy < 0.0f, r.dz < 0.0f};
-  uint32_t ref = S.root_ref;
-  uint32_t sp = 0;
+  const uint32_t neg = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
+  Walk w{S.root_ref, kNoRef, 0};
+  w.park(stk);
   for (;;) {
-    if (!(ref & kLeafBit)) {
-      const float4* n = S.node2 + (size_t)ref * kNode2Stride;
+    while (w.inner()) {
+      const float4* n = S.node2 + (size_t)w.ref * kNode2Stride;
       const float4 a = n[0], b = n[1], cc = n[2];
       const uint4 m = *reinterpret_cast<const uint4*>(n + 3);
       float tl, tr;
       const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, h.t, tl);
       const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, h.t, tr);
       c.visits += 2;
-      const bool right_first = neg[m.z];
+      const bool right_first = (neg >> m.z) & 1u;
       const uint32_t first = right_first ? m.y : m.x, second = right_first ? m.x : m.y;
       const bool hf = right_first ? hr : hl, hs = right_first ? hl : hr;
-      if (hf) {
-        if (hs) {
-          stk.ref[sp * stk.stride] = second;
-          ++sp;
-        }
-        ref = first;
-        continue;
-      }
-      if (hs) {
-        ref = second;
-        continue;
-      }
-    } else {
-      const uint32_t k0 = leaf_first(ref), kn = k0 + leaf_count(ref);
+      w.step(stk, first, second, hf, hs);
+      if (w.all_parked()) break;
+    }
+    while (w.leaf != kNoRef) {
+      const uint32_t k0 = leaf_first(w.leaf), kn = k0 + leaf_count(w.leaf);
       for (uint32_t k = k0; k < kn; ++k) {
         ++c.tests;
         float t, b1, b2;
@@ -430,10 +464,10 @@ __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& 
           h.prim = prim;
         }
       }
+      w.leaf = kNoRef;
+      w.park(stk);
     }
-    if (sp == 0) break;
-    --sp;
-    ref = stk.ref[sp * stk.stride];
+    if (w.ref == kNoRef) break;
   }
   return h;
 }
@@ -461,39 +495,30 @@ __device__ inline bool any32(const DevScene& S, bool use_bvh, const Ray32& r, fl
   ++c.visits;
   if (!box32(r, S.root_lo[0], S.root_lo[1], S.root_lo[2], S.root_hi[0], S.root_hi[1], S.root_hi[2], tmin, tmax, tn))
     return false;
-  uint32_t ref = S.root_ref;
-  uint32_t sp = 0;
+  Walk w{S.root_ref, kNoRef, 0};
+  w.park(stk);
   for (;;) {
-    if (!(ref & kLeafBit)) {
-      const float4* n = S.node2 + (size_t)ref * kNode2Stride;
+    while (w.inner()) {
+      const float4* n = S.node2 + (size_t)w.ref * kNode2Stride;
       const float4 a = n[0], b = n[1], cc = n[2];
       const uint4 m = *reinterpret_cast<const uint4*>(n + 3);
       float tl, tr;
       const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, tmax, tl);
       const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, tmax, tr);
       c.visits += 2;
-      if (hl) {  // Bvh::any visits idx+1 (left) first, pushes the right child
-        if (hr) {
-          stk.ref[sp * stk.stride] = m.y;
-          ++sp;
-        }
-        ref = m.x;
-        continue;
-      }
-      if (hr) {
-        ref = m.y;
-        continue;
-      }
-    } else {
-      const uint32_t k0 = leaf_first(ref), kn = k0 + leaf_count(ref);
+      w.step(stk, m.x, m.y, hl, hr);  // Bvh::any visits idx+1 (left) first, pushes the right child
+      if (w.all_parked()) break;
+    }
+    while (w.leaf != kNoRef) {
+      const uint32_t k0 = leaf_first(w.leaf), kn = k0 + leaf_count(w.leaf);
       for (uint32_t k = k0; k < kn; ++k) {
         ++c.tests;
         if (slot_hit32(S.slot32, k, r, tmin, tmax, t, b1, b2, prim)) return true;
       }
+      w.leaf = kNoRef;
+      w.park(stk);
     }
-    if (sp == 0) break;
-    --sp;
-    ref = stk.ref[sp * stk.stride];
+    if (w.ref == kNoRef) break;
   }
   return false;
 }
