@@ -819,7 +819,7 @@ rtg_status ensure_batch_tables(rtg_context* ctx) {
   return RTG_OK;
 }
 
-int stack_class(uint32_t stack) { return stack <= kStackClasses[0] ? 0 : stack <= kStackClasses[1] ? 1 : 2; }
+int stack_class(uint32_t stack) { return stack_class_of(stack); }
 
 // One batched pass over segments seg[0..n) (n <= kMaxBatchJobs, total paths
 // <= the batch capacity). Adds each segment's counters to its job's stats.

@@ -60,22 +60,28 @@ void trace_impl(const DevScene& S, const double* rays, uint64_t n, int mode, int
 
 void run_chunk(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
                const LaunchHooks* h) {
-  if (stack <= kStackClasses[0]) chunk_impl<20>(S, B, P, s, h);
-  else if (stack <= kStackClasses[1]) chunk_impl<32>(S, B, P, s, h);
-  else chunk_impl<64>(S, B, P, s, h);
+  switch (stack_class_of(stack)) {
+    case 0: chunk_impl<19>(S, B, P, s, h); break;
+    case 1: chunk_impl<31>(S, B, P, s, h); break;
+    default: chunk_impl<63>(S, B, P, s, h); break;
+  }
 }
 
 void run_batch(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPlans& L, int depth, uint32_t stack,
                const BatchGrids& g, hipStream_t s, const LaunchHooks* h) {
-  if (stack <= kStackClasses[0]) batch_impl<20>(jobs, B, L, depth, g, s, h);
-  else if (stack <= kStackClasses[1]) batch_impl<32>(jobs, B, L, depth, g, s, h);
-  else batch_impl<64>(jobs, B, L, depth, g, s, h);
+  switch (stack_class_of(stack)) {
+    case 0: batch_impl<19>(jobs, B, L, depth, g, s, h); break;
+    case 1: batch_impl<31>(jobs, B, L, depth, g, s, h); break;
+    default: batch_impl<63>(jobs, B, L, depth, g, s, h); break;
+  }
 }
 
 void batch_occupancy(uint32_t stack, uint32_t* trace, uint32_t* shade, uint32_t* other) {
-  if (stack <= kStackClasses[0]) *trace = occupancy_trace<20>();
-  else if (stack <= kStackClasses[1]) *trace = occupancy_trace<32>();
-  else *trace = occupancy_trace<64>();
+  switch (stack_class_of(stack)) {
+    case 0: *trace = occupancy_trace<19>(); break;
+    case 1: *trace = occupancy_trace<31>(); break;
+    default: *trace = occupancy_trace<63>(); break;
+  }
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_shade<float>, kBlock, 0) != hipSuccess) n = 1;
   *shade = (uint32_t)std::max(1, n);
@@ -88,9 +94,11 @@ void batch_occupancy(uint32_t stack, uint32_t* trace, uint32_t* shade, uint32_t*
 
 void run_trace32(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,
                  unsigned long long* stats, uint32_t stack, hipStream_t s, const LaunchHooks* h) {
-  if (stack <= kStackClasses[0]) trace_impl<20>(S, rays, n, mode, use_bvh, hits, stats, s, h);
-  else if (stack <= kStackClasses[1]) trace_impl<32>(S, rays, n, mode, use_bvh, hits, stats, s, h);
-  else trace_impl<64>(S, rays, n, mode, use_bvh, hits, stats, s, h);
+  switch (stack_class_of(stack)) {
+    case 0: trace_impl<19>(S, rays, n, mode, use_bvh, hits, stats, s, h); break;
+    case 1: trace_impl<31>(S, rays, n, mode, use_bvh, hits, stats, s, h); break;
+    default: trace_impl<63>(S, rays, n, mode, use_bvh, hits, stats, s, h); break;
+  }
 }
 
 }  // namespace rtg

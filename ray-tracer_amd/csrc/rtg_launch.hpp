@@ -7,8 +7,16 @@
 
 namespace rtg {
 
-// Stack classes of the FP32 traversal kernels (entries per lane in LDS).
-constexpr uint32_t kStackClasses[3] = {20, 32, 64};
+// LDS stack classes of the FP32 traversal kernels (entries per lane). A
+// lane's stack top sits in a register and the LDS slots it uses lie below
+// the BVH depth, so a scene whose BVH depth + 1 is `stack` fits a class of
+// stack - 1 entries. 19 entries keep a traversal workgroup under 10 KB of
+// LDS: 16 workgroups (8 waves per SIMD) fit a CU.
+constexpr uint32_t kStackClasses[3] = {19, 31, 63};
+inline int stack_class_of(uint32_t stack) {
+  const uint32_t need = stack > 1 ? stack - 1 : 1;
+  return need <= kStackClasses[0] ? 0 : need <= kStackClasses[1] ? 1 : 2;
+}
 
 // Optional per-launch hook (profiling): called before and after every
 // kernel launch with the RTG_K_* kind.

@@ -378,30 +378,44 @@ constexpr uint32_t kNoRef = 0xFFFFFFFFu;
 struct Walk {
   uint32_t ref;   // next node to visit (kNoRef: traversal done)
   uint32_t leaf;  // parked leaf (kNoRef: none)
-  uint32_t sp;    // stack depth
+  uint32_t n;     // stack entries: the top in `top`, the rest in LDS slots 0 .. n-2
+  uint32_t top;
 
   __device__ __forceinline__ bool inner() const { return !(ref & kLeafBit); }
-  __device__ __forceinline__ uint32_t pop(LdsStack s) {
-    if (sp == 0) return kNoRef;
-    --sp;
-    return s.ref[sp * kStackStride];
+  // Entry n-2, the top after a pop (any slot's contents when n < 2).
+  __device__ __forceinline__ uint32_t below(LdsStack s) const { return s.ref[(n >= 2 ? n - 2 : 0) * kStackStride]; }
+  __device__ __forceinline__ uint32_t pop(uint32_t under) {
+    const uint32_t r = n ? top : kNoRef;
+    top = under;
+    n -= n ? 1u : 0u;
+    return r;
   }
   // Parks ref if it is a leaf and no leaf is parked, continuing from the stack.
   __device__ __forceinline__ void park(LdsStack s) {
     if (leaf == kNoRef && ref != kNoRef && (ref & kLeafBit)) {
       leaf = ref;
-      ref = pop(s);
+      ref = pop(below(s));
     }
   }
   // One inner node with children `first` (visited first) and `second` and
-  // whether their boxes were hit. The push slot is written unconditionally
-  // and kept only when both are hit: at an inner node of depth d the stack
-  // holds at most d entries, so the slot lies below the BVH depth + 1.
-  __device__ __forceinline__ void step(LdsStack s, uint32_t first, uint32_t second, bool hf, bool hs) {
-    s.ref[sp * kStackStride] = second;
-    sp += (hf && hs) ? 1u : 0u;
-    ref = hf ? first : hs ? second : pop(s);
-    park(s);
+  // whether their boxes were hit; `under` = below(), read before the node.
+  // Branch-free: the top is spilled to its slot on every step (kept only by
+  // a push; at an inner node of depth d the stack holds at most d entries,
+  // so the slot lies below the BVH depth), and a child leaf is parked when
+  // none is, continuing with the other child or the stack.
+  __device__ __forceinline__ void step(LdsStack s, uint32_t under, uint32_t first, uint32_t second, bool hf,
+                                       bool hs) {
+    s.ref[(n ? n - 1 : 0) * kStackStride] = top;
+    const uint32_t c = hf ? first : second;
+    const bool both = hf && hs, any = hf || hs;
+    const bool park_c = any && leaf == kNoRef && (c & kLeafBit);
+    const bool push = both && !park_c;
+    const bool popping = !any || (park_c && !both);
+    const uint32_t popped = n ? top : kNoRef;
+    leaf = park_c ? c : leaf;
+    ref = popping ? popped : (park_c ? second : c);
+    top = push ? second : popping ? under : top;
+    n = push ? n + 1 : (popping && n) ? n - 1 : n;
   }
   // Every lane still descending has a leaf parked (a wave-wide vote).
   __device__ __forceinline__ bool all_parked() const { return __builtin_amdgcn_ballot_w64(leaf == kNoRef) == 0; }
@@ -434,10 +448,11 @@ __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& 
   if (!box32(r, S.root_lo[0], S.root_lo[1], S.root_lo[2], S.root_hi[0], S.root_hi[1], S.root_hi[2], tmin, h.t, tn))
     return h;
   const uint32_t neg = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
-  Walk w{S.root_ref, kNoRef, 0};
+  Walk w{S.root_ref, kNoRef, 0, 0};
   w.park(stk);
   for (;;) {
     while (w.inner()) {
+      const uint32_t under = w.below(stk);
       const float4* n = S.node2 + (size_t)w.ref * kNode2Stride;
       const float4 a = n[0], b = n[1], cc = n[2];
       const uint4 m = *reinterpret_cast<const uint4*>(n + 3);
@@ -448,9 +463,10 @@ __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& 
       const bool right_first = (neg >> m.z) & 1u;
       const uint32_t first = right_first ? m.y : m.x, second = right_first ? m.x : m.y;
       const bool hf = right_first ? hr : hl, hs = right_first ? hl : hr;
-      w.step(stk, first, second, hf, hs);
+      w.step(stk, under, first, second, hf, hs);
       if (w.all_parked()) break;
     }
+    w.park(stk);  // a leaf popped after a miss
     while (w.leaf != kNoRef) {
       const uint32_t k0 = leaf_first(w.leaf), kn = k0 + leaf_count(w.leaf);
       for (uint32_t k = k0; k < kn; ++k) {
@@ -495,10 +511,11 @@ __device__ inline bool any32(const DevScene& S, bool use_bvh, const Ray32& r, fl
   ++c.visits;
   if (!box32(r, S.root_lo[0], S.root_lo[1], S.root_lo[2], S.root_hi[0], S.root_hi[1], S.root_hi[2], tmin, tmax, tn))
     return false;
-  Walk w{S.root_ref, kNoRef, 0};
+  Walk w{S.root_ref, kNoRef, 0, 0};
   w.park(stk);
   for (;;) {
     while (w.inner()) {
+      const uint32_t under = w.below(stk);
       const float4* n = S.node2 + (size_t)w.ref * kNode2Stride;
       const float4 a = n[0], b = n[1], cc = n[2];
       const uint4 m = *reinterpret_cast<const uint4*>(n + 3);
@@ -506,9 +523,10 @@ __device__ inline bool any32(const DevScene& S, bool use_bvh, const Ray32& r, fl
       const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, tmax, tl);
       const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, tmax, tr);
       c.visits += 2;
-      w.step(stk, m.x, m.y, hl, hr);  // Bvh::any visits idx+1 (left) first, pushes the right child
+      w.step(stk, under, m.x, m.y, hl, hr);  // Bvh::any visits idx+1 (left) first, pushes the right child
       if (w.all_parked()) break;
     }
+    w.park(stk);  // a leaf popped after a miss
     while (w.leaf != kNoRef) {
       const uint32_t k0 = leaf_first(w.leaf), kn = k0 + leaf_count(w.leaf);
       for (uint32_t k = k0; k < kn; ++k) {
