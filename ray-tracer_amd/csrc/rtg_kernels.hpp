@@ -124,18 +124,27 @@ __device__ __forceinline__ AppendSlots block_append2(uint32_t* counts_a, uint32_
 // kSubQueues counts once into LDS as a prefix sum. Returns false past the end.
 struct QueueView {
   uint32_t total;
+  bool past_end;  // the block has no entries: it returns before any barrier
 };
-__device__ __forceinline__ QueueView load_queue(const uint32_t* counts, uint32_t* pre) {
-  if (threadIdx.x == 0) {
-    uint32_t s = 0;
-    for (int k = 0; k < kSubQueues; ++k) {
-      pre[k] = s;
-      s += counts[k];
-    }
-    pre[kSubQueues] = s;
+// Every wave sums the kSubQueues sub-queue sizes itself (one load per lane
+// and a shuffle scan), so both waves of a block agree on the total without
+// LDS or a barrier, and blocks past the end of the queue (the grid covers
+// every path, the queue shrinks with each bounce) exit at once. Other blocks
+// publish the exclusive prefix sums in pre[] for queue_slot.
+__device__ __forceinline__ QueueView load_queue(const uint32_t* counts, uint32_t* pre, uint32_t lb) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t c = lane < (uint32_t)kSubQueues ? counts[lane] : 0u;
+  uint32_t x = c;
+  for (int off = 1; off < kSubQueues; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= (uint32_t)off) x += y;
   }
+  const uint32_t total = __shfl(x, kSubQueues - 1, 64);
+  if (lb * (uint32_t)kBlock >= total) return QueueView{total, true};
+  if (threadIdx.x < (uint32_t)kSubQueues) pre[threadIdx.x] = x - c;
+  if (threadIdx.x == 0) pre[kSubQueues] = total;
   __syncthreads();
-  return QueueView{pre[kSubQueues]};
+  return QueueView{total, false};
 }
 __device__ __forceinline__ uint32_t queue_slot(const uint32_t* pre, uint32_t sub_cap, uint32_t i) {
   int lo = 0, hi = kSubQueues - 1;  // last k with pre[k] <= i
@@ -325,7 +334,8 @@ __device__ __forceinline__ uint32_t active_pid(const PathBuffers<T>& B, const Ch
 template <class T, int STACK>
 __device__ __forceinline__ void extend_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P, int b,
                                             uint32_t lb, uint32_t* sref, uint32_t* pre) {
-  const QueueView qv = load_queue(P.counts + (size_t)b * kSubQueues, pre);
+  const QueueView qv = load_queue(P.counts + (size_t)b * kSubQueues, pre, lb);
+  if (qv.past_end) return;
   const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
   if (i < qv.total) {
@@ -355,7 +365,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene S, PathBuffers<T> B,
 template <class T>
 __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P, int b,
                                            uint32_t lb, uint32_t* pre) {
-  const QueueView qv = load_queue(P.counts + (size_t)b * kSubQueues, pre);
+  const QueueView qv = load_queue(P.counts + (size_t)b * kSubQueues, pre, lb);
+  if (qv.past_end) return;
   const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
   bool alive = false, shadow = false;
@@ -441,7 +452,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, PathBuffers<T> B, 
 template <class T, int STACK>
 __device__ __forceinline__ void shadow_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P, int b,
                                             uint32_t lb, uint32_t* sref, uint32_t* pre) {
-  const QueueView qv = load_queue(P.counts + P.shadow_counts + (size_t)b * kSubQueues, pre);
+  const QueueView qv = load_queue(P.counts + P.shadow_counts + (size_t)b * kSubQueues, pre, lb);
+  if (qv.past_end) return;
   const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
   if (i < qv.total) {
