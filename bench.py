@@ -303,7 +303,9 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches": ext["launches"],
                 "node_visits_per_ray": prof["closest_visits"] / max(1, prof["closest_rays"]),
-                "tests_per_ray": prof["closest_tests"] / max(1, prof["closest_rays"])}
+                "tests_per_ray": prof["closest_tests"] / max(1, prof["closest_rays"]),
+                "shadow_node_visits_per_ray": prof["shadow_visits"] / max(1, prof["shadow_rays"]),
+                "shadow_tests_per_ray": prof["shadow_tests"] / max(1, prof["shadow_rays"])}
     kernel_ms = {k: round(v["ms"] / args.steps, 3) for k, v in prof.items() if isinstance(v, dict)}
     dev.release()
     acc.release()
@@ -322,7 +324,10 @@ def main():
                                f"{info['planes']} plane, {cfg.width}x{cfg.height}, {spp} spp, depth {cfg.max_depth}, "
                                f"seed {cfg.seed}, progressive passes",
                    "bvh_nodes": info["nodes"], "bvh_depth": info["max_depth"], "scene_prep_ms": round(prep_ms, 1),
-                   "rays_per_step": rays // args.steps, "kernel_ms_per_step": kernel_ms},
+                   "rays_per_step": rays // args.steps,
+                   "closest_rays_per_step": prof["closest_rays"] // args.steps,
+                   "shadow_rays_per_step": (rays - prof["closest_rays"]) // args.steps,
+                   "kernel_ms_per_step": kernel_ms},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batch,

@@ -13,7 +13,9 @@
 // layout: both child boxes are tested from one 64-byte record against the
 // current closest t, and only the far child's reference is pushed; a popped
 // child is entered without a second box test (its box was already tested,
-// and any primitive in it still has to beat the closest t).
+// and any primitive in it still has to beat the closest t). The FP32 walk
+// parks leaves (see Walk), so a ray may test a leaf after nodes that follow
+// it in depth-first order; every such choice depends on the ray alone.
 #pragma once
 #include "rtg_math.hpp"
 #include "rtg_scene.hpp"
@@ -369,12 +371,22 @@ struct LdsStack {
 // count never exceeds 2^25), and it has the leaf bit, so inner() is false.
 constexpr uint32_t kNoRef = 0xFFFFFFFFu;
 
-// Traversal state of one ray, walked as two loops per round (Aila & Laine,
-// "Understanding the Efficiency of Ray Traversal on GPUs", 2009): descend
-// inner nodes until every lane of the wave has reached a leaf, parking the
-// first leaf a lane meets and continuing from its stack, then test the
-// parked leaves. Lanes that reach a leaf early keep descending instead of
-// idling while the rest of the wave finishes its inner nodes.
+// Traversal state of one ray, walked as two loops per round (after Aila &
+// Laine, "Understanding the Efficiency of Ray Traversal on GPUs", 2009):
+// descend inner nodes, parking the first leaf met and continuing from the
+// stack, then test the parked leaf. A lane that reaches a leaf early keeps
+// descending instead of idling while the rest of its wave is still on inner
+// nodes; it leaves the descent at its next leaf or once it has taken
+// kDescent steps in the round. The paper ends the descent by a wave-wide vote
+// (every lane has a leaf parked), which is ~3.5 % faster on the torus
+// workload but makes the order in which a ray meets nodes and leaves depend
+// on the other rays of its wave: then which of two hits at equal t a ray
+// keeps (a box whose computed entry rounds past a hit on its face is culled
+// or not depending on when that hit was found), and its node-visit counts,
+// would depend on how paths are grouped into waves. With a rule of the lane's
+// own, images and counters are independent of chunking, batching and queue
+// order.
+constexpr uint32_t kDescent = 16;
 struct Walk {
   uint32_t ref;   // next node to visit (kNoRef: traversal done)
   uint32_t leaf;  // parked leaf (kNoRef: none)
@@ -417,8 +429,8 @@ struct Walk {
     top = push ? second : popping ? under : top;
     n = push ? n + 1 : (popping && n) ? n - 1 : n;
   }
-  // Every lane still descending has a leaf parked (a wave-wide vote).
-  __device__ __forceinline__ bool all_parked() const { return __builtin_amdgcn_ballot_w64(leaf == kNoRef) == 0; }
+  // Leaves the descent: a leaf is parked and `steps` steps were taken.
+  __device__ __forceinline__ bool descended(uint32_t steps) const { return leaf != kNoRef && steps >= kDescent; }
 };
 
 __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& r, float tmin, float tmax,
@@ -451,7 +463,7 @@ __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& 
   Walk w{S.root_ref, kNoRef, 0, 0};
   w.park(stk);
   for (;;) {
-    while (w.inner()) {
+    for (uint32_t steps = 1; w.inner(); ++steps) {
       const uint32_t under = w.below(stk);
       const float4* n = S.node2 + (size_t)w.ref * kNode2Stride;
       const float4 a = n[0], b = n[1], cc = n[2];
@@ -464,7 +476,7 @@ __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& 
       const uint32_t first = right_first ? m.y : m.x, second = right_first ? m.x : m.y;
       const bool hf = right_first ? hr : hl, hs = right_first ? hl : hr;
       w.step(stk, under, first, second, hf, hs);
-      if (w.all_parked()) break;
+      if (w.descended(steps)) break;
     }
     w.park(stk);  // a leaf popped after a miss
     while (w.leaf != kNoRef) {
@@ -514,7 +526,7 @@ __device__ inline bool any32(const DevScene& S, bool use_bvh, const Ray32& r, fl
   Walk w{S.root_ref, kNoRef, 0, 0};
   w.park(stk);
   for (;;) {
-    while (w.inner()) {
+    for (uint32_t steps = 1; w.inner(); ++steps) {
       const uint32_t under = w.below(stk);
       const float4* n = S.node2 + (size_t)w.ref * kNode2Stride;
       const float4 a = n[0], b = n[1], cc = n[2];
@@ -524,7 +536,7 @@ __device__ inline bool any32(const DevScene& S, bool use_bvh, const Ray32& r, fl
       const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, tmax, tr);
       c.visits += 2;
       w.step(stk, under, m.x, m.y, hl, hr);  // Bvh::any visits idx+1 (left) first, pushes the right child
-      if (w.all_parked()) break;
+      if (w.descended(steps)) break;
     }
     w.park(stk);  // a leaf popped after a miss
     while (w.leaf != kNoRef) {

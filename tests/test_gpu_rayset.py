@@ -78,3 +78,43 @@ def test_f32_mismatches_are_rare(ctx, torus):
           f"{(dt <= 1e-5).mean():.6f} of hits within 1e-5", flush=True)
     assert off.max() <= 4e-6
     assert (dt <= 1e-5).mean() >= 0.9999
+
+
+def _unpermute(hits, perm):
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    return hits[inv]
+
+
+def test_f32_hits_do_not_depend_on_wave_neighbours(ctx, torus):
+    """The FP32 walk parks leaves and keeps descending; when a ray tests its
+    leaves must depend on the ray alone, not on the other rays of its wave.
+    Otherwise the hit kept among hits at (nearly) equal t can change: a box
+    whose computed entry rounds past a hit on its face is culled or not
+    depending on when that hit was found. Rays aimed at mesh vertices (each
+    shared by six triangles, on the faces of leaf boxes) make such cases;
+    every ray must get the same hit (id, t, face) and any-hit answer when the
+    rays are traced in another order, and the traversal counters must agree."""
+    hs, dev, rays = torus
+    prims = orc.arrays(hs.desc)["prims"].view(np.float64).reshape(-1, 20)
+    rng = np.random.default_rng(11)
+    target = prims[rng.integers(0, len(prims), 1 << 20), 2:5]
+    u = rng.normal(size=target.shape)
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    aimed = np.zeros((len(target), 8))
+    aimed[:, 0:3] = target + 0.6 * u
+    aimed[:, 3:6] = -u
+    aimed[:, 6], aimed[:, 7] = 1e-6, np.inf
+    for name, sub in (("random", rays[:1 << 21]), ("vertex-aimed", aimed)):
+        perm = np.random.default_rng(3).permutation(len(sub))
+        for mode in (0, 1):
+            a, sa = ctx.trace(dev, sub, mode=mode, precision=rtg.PRECISION_F32)
+            b, sb = ctx.trace(dev, sub[perm], mode=mode, precision=rtg.PRECISION_F32)
+            b = _unpermute(b, perm)
+            assert (sa.node_visits, sa.prim_tests) == (sb.node_visits, sb.prim_tests), (name, mode)
+            print(f"{name} mode {mode}: {int((a['flags'] & 1).sum())} hits, "
+                  f"{int((a['prim'] != b['prim']).sum())} id differences", flush=True)
+            assert np.array_equal(a["flags"], b["flags"]), (name, mode)
+            if mode == 0:
+                assert np.array_equal(a["prim"], b["prim"]), name
+                assert np.array_equal(a["t"].view(np.uint64), b["t"].view(np.uint64)), name
