@@ -814,27 +814,27 @@ struct BatchPlans {
 __device__ __forceinline__ uint32_t blocks_of(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 // Job of virtual block vb: the last j with plan[j] <= vb (jobs without
-// blocks repeat their successor's start and are skipped).
-__device__ __forceinline__ uint32_t job_of(const uint32_t* plan, uint32_t n_jobs, uint32_t vb) {
-  int lo = 0, hi = (int)n_jobs - 1;
+// blocks repeat their successor's start and are skipped). vb is uniform
+// across the workgroup, so the search runs on scalar loads of the plan
+// (read-only during the launch; it stays in the scalar cache) and the
+// workgroup keeps its LDS for traversal stacks.
+__device__ __forceinline__ uint32_t job_of(const uint32_t* __restrict__ plan, uint32_t n_jobs, uint32_t vb) {
+  uint32_t lo = 0, hi = n_jobs - 1;
   while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
+    const uint32_t mid = (lo + hi + 1) >> 1;
     if (plan[mid] <= vb) lo = mid;
     else hi = mid - 1;
   }
-  return (uint32_t)__builtin_amdgcn_readfirstlane(lo);
+  return __builtin_amdgcn_readfirstlane(lo);
 }
 
 // Runs f(job, local block) over every virtual block of a plan.
 template <class F>
-__device__ __forceinline__ void batch_loop(const uint32_t* __restrict__ plan_g, uint32_t n_jobs, uint32_t* s_plan,
-                                           F&& f) {
-  for (uint32_t t = threadIdx.x; t <= n_jobs; t += blockDim.x) s_plan[t] = plan_g[t];
-  __syncthreads();
-  const uint32_t total = s_plan[n_jobs];
+__device__ __forceinline__ void batch_loop(const uint32_t* __restrict__ plan, uint32_t n_jobs, F&& f) {
+  const uint32_t total = __builtin_amdgcn_readfirstlane(plan[n_jobs]);
   for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
-    const uint32_t j = job_of(s_plan, n_jobs, vb);
-    f(j, vb - s_plan[j]);
+    const uint32_t j = job_of(plan, n_jobs, vb);
+    f(j, vb - __builtin_amdgcn_readfirstlane(plan[j]));
   }
 }
 
@@ -875,8 +875,7 @@ __global__ __launch_bounds__(kMaxBatchJobs) void k_batch_plan(const BatchJob* __
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_batch_generate(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
                                                            BatchPlans L) {
-  __shared__ uint32_t s_plan[kMaxBatchJobs + 1];
-  batch_loop(L.gen, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) { generate_body<T>(jobs[j].S, B, jobs[j].P, lb); });
+  batch_loop(L.gen, L.n_jobs, [&](uint32_t j, uint32_t lb) { generate_body<T>(jobs[j].S, B, jobs[j].P, lb); });
 }
 
 template <class T, int STACK>
@@ -884,8 +883,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_extend(const BatchJob* __restr
                                                          BatchPlans L, int b) {
   __shared__ uint32_t sref[STACK * kBlock];
   __shared__ uint32_t pre[kSubQueues + 1];
-  __shared__ uint32_t s_plan[kMaxBatchJobs + 1];
-  batch_loop(L.act, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) {
+  batch_loop(L.act, L.n_jobs, [&](uint32_t j, uint32_t lb) {
     extend_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, pre);
   });
 }
@@ -894,8 +892,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_batch_shade(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
                                                         BatchPlans L, int b) {
   __shared__ uint32_t pre[kSubQueues + 1];
-  __shared__ uint32_t s_plan[kMaxBatchJobs + 1];
-  batch_loop(L.act, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) { shade_body<T>(jobs[j].S, B, jobs[j].P, b, lb, pre); });
+  batch_loop(L.act, L.n_jobs, [&](uint32_t j, uint32_t lb) { shade_body<T>(jobs[j].S, B, jobs[j].P, b, lb, pre); });
 }
 
 template <class T, int STACK>
@@ -903,8 +900,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_shadow(const BatchJob* __restr
                                                          BatchPlans L, int b) {
   __shared__ uint32_t sref[STACK * kBlock];
   __shared__ uint32_t pre[kSubQueues + 1];
-  __shared__ uint32_t s_plan[kMaxBatchJobs + 1];
-  batch_loop(L.shd, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) {
+  batch_loop(L.shd, L.n_jobs, [&](uint32_t j, uint32_t lb) {
     shadow_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, pre);
   });
 }
@@ -912,9 +908,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_shadow(const BatchJob* __restr
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_batch_accumulate(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
                                                              BatchPlans L) {
-  __shared__ uint32_t s_plan[kMaxBatchJobs + 1];
   __shared__ T tile[kAccLds<T>];
-  batch_loop(L.acc, L.n_jobs, s_plan, [&](uint32_t j, uint32_t lb) { accumulate_body<T>(B, jobs[j].P, lb, tile); });
+  batch_loop(L.acc, L.n_jobs, [&](uint32_t j, uint32_t lb) { accumulate_body<T>(B, jobs[j].P, lb, tile); });
 }
 
 }  // namespace rtg
