@@ -48,9 +48,17 @@ struct rtg_context {
   uint32_t cap[3] = {0, 0, 0};
   // Batches: device job table and plans, pinned host staging of both.
   BatchJob* batch_jobs = nullptr;
-  uint32_t* batch_plans = nullptr;  // 4 x (kMaxBatchJobs + 1): gen, acc, act, shd
-  void* batch_host = nullptr;       // job table + gen/acc plans, pinned
+  uint32_t* batch_plans = nullptr;  // job count, then 4 x (kMaxBatchJobs + 1): gen, acc, act, shd
+  void* batch_host = nullptr;       // job table + job count + gen/acc plans, pinned
   uint32_t batch_occ[3][3] = {};    // per stack class: trace, shade, other workgroups per CU
+  // Captured batch passes, one per (max depth, stack class): the launches of
+  // a pass replay as one graph. Their arguments point into the batch
+  // buffers, so they are dropped whenever those are reallocated.
+  struct BatchGraph {
+    int depth, cls;
+    hipGraphExec_t exec;
+  };
+  std::vector<BatchGraph> batch_graphs;
   uint8_t* rgb_dev = nullptr;       // rtg_resolve staging
   size_t rgb_cap = 0;
   // Profiling: an event pair per launch, resolved when the call completes.
@@ -61,6 +69,11 @@ struct rtg_context {
   rtg_profile profile{};
   LaunchHooks hooks{};
 };
+
+void drop_batch_graphs(rtg_context* ctx) {
+  for (auto& g : ctx->batch_graphs) hipGraphExecDestroy(g.exec);
+  ctx->batch_graphs.clear();
+}
 
 // Device arrays of a scene, shared by the handles made from it with other
 // cameras (rtg_scene_with_camera); freed with the last of them.
@@ -279,6 +292,7 @@ rtg_status rtg_close(rtg_context* ctx) {
   if (!ctx) return RTG_OK;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
+  drop_batch_graphs(ctx);
   for (void* p : ctx->path_mem)
     if (p) hipFree(p);
   if (ctx->stats) hipFree(ctx->stats);
@@ -621,6 +635,7 @@ namespace {
 template <class T>
 rtg_status ensure_paths(rtg_context* ctx, int which, uint32_t cap, PathBuffers<T>& B) {
   if (ctx->cap[which] >= cap) return RTG_OK;
+  if (which == kBatchBuffers) drop_batch_graphs(ctx);
   if (ctx->path_mem[which]) {
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     HIP_TRY(ctx, hipFree(ctx->path_mem[which]));
@@ -811,11 +826,19 @@ bool batchable(const rtg_job& j) {
          j.settings.max_depth <= kBatchDepth;
 }
 
+// Batch passes replay as captured graphs unless RTG_BATCH_GRAPHS=0 (direct
+// launches; also used while per-launch profiling is on).
+bool batch_graphs_enabled() {
+  const char* e = std::getenv("RTG_BATCH_GRAPHS");
+  return !(e && e[0] == '0');
+}
+
 rtg_status ensure_batch_tables(rtg_context* ctx) {
   if (ctx->batch_jobs) return RTG_OK;
   HIP_TRY(ctx, hipMalloc(&ctx->batch_jobs, sizeof(BatchJob) * kMaxBatchJobs));
-  HIP_TRY(ctx, hipMalloc(&ctx->batch_plans, sizeof(uint32_t) * 4 * (kMaxBatchJobs + 1)));
-  HIP_TRY(ctx, hipHostMalloc(&ctx->batch_host, sizeof(BatchJob) * kMaxBatchJobs + sizeof(uint32_t) * 2 * (kMaxBatchJobs + 1)));
+  HIP_TRY(ctx, hipMalloc(&ctx->batch_plans, sizeof(uint32_t) * (1 + 4 * (kMaxBatchJobs + 1))));
+  HIP_TRY(ctx, hipHostMalloc(&ctx->batch_host,
+                             sizeof(BatchJob) * kMaxBatchJobs + sizeof(uint32_t) * (1 + 2 * (kMaxBatchJobs + 1))));
   return RTG_OK;
 }
 
@@ -826,8 +849,10 @@ int stack_class(uint32_t stack) { return stack_class_of(stack); }
 rtg_status run_group(rtg_context* ctx, const rtg_job* jobs, const Segment* seg, uint32_t n,
                      std::vector<rtg_stats>& acc_stats) {
   auto* hj = static_cast<BatchJob*>(ctx->batch_host);
-  auto* hgen = reinterpret_cast<uint32_t*>(hj + kMaxBatchJobs);
+  auto* hn = reinterpret_cast<uint32_t*>(hj + kMaxBatchJobs);
+  uint32_t* hgen = hn + 1;
   uint32_t* hacc = hgen + (kMaxBatchJobs + 1);
+  hn[0] = n;
   const PathBuffers<float>& B = ctx->pbb;
   uint64_t pid = 0, q = 0;
   uint32_t gen_blocks = 0, acc_blocks = 0, path_blocks = 0, stack = 1;
@@ -866,19 +891,47 @@ rtg_status run_group(rtg_context* ctx, const rtg_job* jobs, const Segment* seg, 
     return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_batch: internal error: group exceeds the batch buffers");
   uint32_t* plans = ctx->batch_plans;
   HIP_TRY(ctx, hipMemcpyAsync(ctx->batch_jobs, hj, sizeof(BatchJob) * n, hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(ctx, hipMemcpyAsync(plans, hgen, sizeof(uint32_t) * 2 * (kMaxBatchJobs + 1), hipMemcpyHostToDevice,
+  HIP_TRY(ctx, hipMemcpyAsync(plans, hn, sizeof(uint32_t) * (1 + 2 * (kMaxBatchJobs + 1)), hipMemcpyHostToDevice,
                               ctx->stream));
-  BatchPlans L{plans, plans + (kMaxBatchJobs + 1), plans + 2 * (kMaxBatchJobs + 1), plans + 3 * (kMaxBatchJobs + 1), n};
-  uint32_t* occ = ctx->batch_occ[stack_class(stack)];
+  constexpr uint32_t kP = kMaxBatchJobs + 1;
+  const BatchPlans L{plans, plans + 1, plans + 1 + kP, plans + 1 + 2 * kP, plans + 1 + 3 * kP};
+  const int cls = stack_class(stack);
+  uint32_t* occ = ctx->batch_occ[cls];
   if (!occ[0]) batch_occupancy(stack, &occ[0], &occ[1], &occ[2]);
   const uint32_t cu = (uint32_t)std::max(1, ctx->n_cu);
+  // Persistent grids: resident workgroups walk the virtual blocks. A direct
+  // launch trims them to the work; a graph keeps them fixed (its launches
+  // replay with the arguments they were captured with) and the surplus
+  // workgroups find no block and exit.
+  const bool graph = !ctx->profiling && batch_graphs_enabled();
   BatchGrids g;
-  g.generate = std::max(1u, std::min(gen_blocks, occ[2] * cu));
-  g.trace = std::max(1u, std::min(path_blocks, occ[0] * cu));
-  g.shade = std::max(1u, std::min(path_blocks, occ[1] * cu));
-  g.accumulate = std::max(1u, std::min(acc_blocks, occ[2] * cu));
-  run_batch(ctx->batch_jobs, B, L, depth, stack, g, ctx->stream, hooks_of(ctx));
-  HIP_TRY(ctx, hipGetLastError());
+  g.generate = std::max(1u, graph ? occ[2] * cu : std::min(gen_blocks, occ[2] * cu));
+  g.trace = std::max(1u, graph ? occ[0] * cu : std::min(path_blocks, occ[0] * cu));
+  g.shade = std::max(1u, graph ? occ[1] * cu : std::min(path_blocks, occ[1] * cu));
+  g.accumulate = std::max(1u, graph ? occ[2] * cu : std::min(acc_blocks, occ[2] * cu));
+  if (graph) {
+    hipGraphExec_t exec = nullptr;
+    for (const auto& e : ctx->batch_graphs)
+      if (e.depth == depth && e.cls == cls) exec = e.exec;
+    if (!exec) {
+      hipGraph_t gr = nullptr;
+      HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      run_batch(ctx->batch_jobs, B, L, depth, stack, g, ctx->stream, nullptr);
+      const hipError_t le = hipGetLastError();
+      const hipError_t ce = hipStreamEndCapture(ctx->stream, &gr);
+      hipError_t ie = hipSuccess;
+      if (le == hipSuccess && ce == hipSuccess) ie = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+      if (gr) hipGraphDestroy(gr);
+      for (hipError_t e : {le, ce, ie})
+        if (e != hipSuccess)
+          return fail(ctx, RTG_ERR_HIP, std::string("rtg_render_batch: capturing a batch pass: ") + hipGetErrorString(e));
+      ctx->batch_graphs.push_back({depth, cls, exec});
+    }
+    HIP_TRY(ctx, hipGraphLaunch(exec, ctx->stream));
+  } else {
+    run_batch(ctx->batch_jobs, B, L, depth, stack, g, ctx->stream, hooks_of(ctx));
+    HIP_TRY(ctx, hipGetLastError());
+  }
   static thread_local std::vector<unsigned long long> slots;
   slots.assign((size_t)n * kBatchStatSlots * kStatStride, 0);
   HIP_TRY(ctx, hipMemcpyAsync(slots.data(), B.stats, slots.size() * sizeof(unsigned long long),

@@ -804,12 +804,15 @@ constexpr uint32_t kBatchCounts = 2 * kBatchShadowCounts;       // counters per 
 constexpr uint32_t kBatchStatSlots = 32;
 
 struct BatchPlans {
-  const uint32_t* gen;  // written by the host
-  const uint32_t* acc;  // written by the host
-  uint32_t* act;        // active queue of the next extend/shade (k_batch_plan)
-  uint32_t* shd;        // shadow queue of the next shadow step (k_batch_plan)
-  uint32_t n_jobs;
+  const uint32_t* n_jobs;  // the batch's job count, written by the host
+  const uint32_t* gen;     // written by the host
+  const uint32_t* acc;     // written by the host
+  uint32_t* act;           // active queue of the next extend/shade (k_batch_plan)
+  uint32_t* shd;           // shadow queue of the next shadow step (k_batch_plan)
 };
+// The job count is read from device memory, so a captured sequence of batch
+// launches (a graph) replays for any number of jobs.
+__device__ __forceinline__ uint32_t jobs_of(const BatchPlans& L) { return __builtin_amdgcn_readfirstlane(*L.n_jobs); }
 
 __device__ __forceinline__ uint32_t blocks_of(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
@@ -844,8 +847,9 @@ template <class T>
 __global__ __launch_bounds__(kMaxBatchJobs) void k_batch_plan(const BatchJob* __restrict__ jobs, BatchPlans L, int b) {
   __shared__ uint32_t sa[kMaxBatchJobs], ss[kMaxBatchJobs];
   const uint32_t j = threadIdx.x;
+  const uint32_t n_jobs = jobs_of(L);
   uint32_t na = 0, ns = 0;
-  if (j < L.n_jobs) {
+  if (j < n_jobs) {
     const ChunkParams& P = jobs[j].P;
     if (b + 1 < P.max_depth)
       for (int s = 0; s < kSubQueues; ++s) na += P.counts[(size_t)(b + 1) * kSubQueues + s];
@@ -862,7 +866,7 @@ __global__ __launch_bounds__(kMaxBatchJobs) void k_batch_plan(const BatchJob* __
     ss[j] += c;
     __syncthreads();
   }
-  if (j < L.n_jobs) {
+  if (j < n_jobs) {
     L.act[j + 1] = sa[j];
     L.shd[j + 1] = ss[j];
   }
@@ -875,7 +879,7 @@ __global__ __launch_bounds__(kMaxBatchJobs) void k_batch_plan(const BatchJob* __
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_batch_generate(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
                                                            BatchPlans L) {
-  batch_loop(L.gen, L.n_jobs, [&](uint32_t j, uint32_t lb) { generate_body<T>(jobs[j].S, B, jobs[j].P, lb); });
+  batch_loop(L.gen, jobs_of(L), [&](uint32_t j, uint32_t lb) { generate_body<T>(jobs[j].S, B, jobs[j].P, lb); });
 }
 
 template <class T, int STACK>
@@ -883,7 +887,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_extend(const BatchJob* __restr
                                                          BatchPlans L, int b) {
   __shared__ uint32_t sref[STACK * kBlock];
   __shared__ uint32_t pre[kSubQueues + 1];
-  batch_loop(L.act, L.n_jobs, [&](uint32_t j, uint32_t lb) {
+  batch_loop(L.act, jobs_of(L), [&](uint32_t j, uint32_t lb) {
     extend_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, pre);
   });
 }
@@ -892,7 +896,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_batch_shade(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
                                                         BatchPlans L, int b) {
   __shared__ uint32_t pre[kSubQueues + 1];
-  batch_loop(L.act, L.n_jobs, [&](uint32_t j, uint32_t lb) { shade_body<T>(jobs[j].S, B, jobs[j].P, b, lb, pre); });
+  batch_loop(L.act, jobs_of(L), [&](uint32_t j, uint32_t lb) { shade_body<T>(jobs[j].S, B, jobs[j].P, b, lb, pre); });
 }
 
 template <class T, int STACK>
@@ -900,7 +904,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_shadow(const BatchJob* __restr
                                                          BatchPlans L, int b) {
   __shared__ uint32_t sref[STACK * kBlock];
   __shared__ uint32_t pre[kSubQueues + 1];
-  batch_loop(L.shd, L.n_jobs, [&](uint32_t j, uint32_t lb) {
+  batch_loop(L.shd, jobs_of(L), [&](uint32_t j, uint32_t lb) {
     shadow_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, pre);
   });
 }
@@ -909,7 +913,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_batch_accumulate(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
                                                              BatchPlans L) {
   __shared__ T tile[kAccLds<T>];
-  batch_loop(L.acc, L.n_jobs, [&](uint32_t j, uint32_t lb) { accumulate_body<T>(B, jobs[j].P, lb, tile); });
+  batch_loop(L.acc, jobs_of(L), [&](uint32_t j, uint32_t lb) { accumulate_body<T>(B, jobs[j].P, lb, tile); });
 }
 
 }  // namespace rtg

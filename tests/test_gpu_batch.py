@@ -129,6 +129,27 @@ def test_many_small_jobs_in_several_groups(ctx, specs, monkeypatch):
         assert stats[i].rays == st1.rays and stats[i].node_visits == st1.node_visits
 
 
+def test_graph_replays_equal_direct_launches(ctx, specs, monkeypatch):
+    """A batch pass replays as a captured graph (one per max depth and stack
+    class, reused across calls). Replays in a row, with eager renders in
+    between, must each equal the same batch launched kernel by kernel
+    (RTG_BATCH_GRAPHS=0), and a different job mix must reuse the graph."""
+    for rnd, mix in enumerate((specs, specs[:3], specs, specs[2:])):
+        monkeypatch.setenv("RTG_BATCH_GRAPHS", "1")
+        res_g, (st_g,) = batched(ctx, mix, [(rnd, rnd + 2)])
+        o, cfg, spp = mix[0]
+        solo(ctx, o, cfg, 0, 1)  # eager work between the replays
+        monkeypatch.setenv("RTG_BATCH_GRAPHS", "0")
+        res_d, (st_d,) = batched(ctx, mix, [(rnd, rnd + 2)])
+        for i, ((sg, cg), (sd, cd)) in enumerate(zip(res_g, res_d)):
+            assert np.array_equal(cg, cd), (rnd, i)
+            assert np.array_equal(sg.view(np.uint64), sd.view(np.uint64)), (rnd, i)
+        assert len(st_g) == len(st_d) and any(c.any() for _, c in res_g)
+        for i, (a, b) in enumerate(zip(st_g, st_d)):
+            for k in COUNTERS:
+                assert getattr(a, k) == getattr(b, k), (rnd, i, k)
+
+
 def test_bad_jobs_are_rejected(ctx, specs):
     o, cfg, _ = specs[0]
     dev = ctx.scene(o.desc)
