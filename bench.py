@@ -166,7 +166,7 @@ def batch_leg(rtg, seed=4, count=64):
             "speedup": plain_s / batched_s, "identical": identical and rays_p == rays_b}
 
 
-def serve_leg(rtg, qps=6.0, seconds=10.0, seed=5, burst=300):
+def serve_leg(rtg, qps=6.0, seconds=10.0, seed=5, burst=300, tick_paths=0):
     """The served workload: Poisson arrivals at `qps` for `seconds` over 16
     resident scenes of the batched workload. 80 % previews (128^2, 1-4 spp,
     depth 4, one pass), 20 % finals (512^2, 768^2 or 1024^2, 256 spp, the
@@ -179,7 +179,7 @@ def serve_leg(rtg, qps=6.0, seconds=10.0, seed=5, burst=300):
     specs = make_batch.build(seed, 16)
     hs = [rtg.HostScene(json.dumps(j)) for j in specs]
     depth = [int(j["render"]["max_depth"]) for j in specs]
-    srv = rtg.Server(0, 0, 0)
+    srv = rtg.Server(0, 0, tick_paths)
     for h in hs:
         srv.add_scene(h)
     rng = np.random.default_rng(seed)
@@ -240,6 +240,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
     ap.add_argument("--qps", type=float, default=6.0, help="served workload: request arrival rate")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="served workload: trace length")
+    ap.add_argument("--tick-paths", type=int, default=0,
+                    help="served workload: final-render paths per tick (0: tuned per tick)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (3 times): the reference kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -311,7 +313,8 @@ def main():
     acc.release()
     ctx.close()
     batch = None if args.no_batch else batch_leg(rtg)
-    served = None if args.no_serve else serve_leg(rtg, qps=args.qps, seconds=args.serve_seconds)
+    served = None if args.no_serve else serve_leg(rtg, qps=args.qps, seconds=args.serve_seconds,
+                                                     tick_paths=args.tick_paths)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(text, cfg.width, cfg.height)

@@ -109,7 +109,7 @@ int rth_render(rth_scene* s, int32_t device, int32_t precision, int32_t progress
  * scheduled in ticks of batched passes (rtg_render_batch). Previews render
  * all samples of their pass in the next tick; finals advance their
  * progressive passes (1, 1, 2, 4, ... capped at max_pass, renderer.cpp:66-69)
- * by at most tick_paths paths per tick; requests are admitted first come
+ * in slices sized per tick (see rts_create); requests are admitted first come
  * first served while their accumulation buffers fit budget_bytes. It has no
  * counterpart in the reference, whose editor backend starts one `rt` process
  * per job (web/backend/jobs.go:130-259). Errors: 1 + rth_last_error(). */
@@ -139,7 +139,8 @@ typedef struct rts_stats {
   uint64_t peak_accum_bytes;
 } rts_stats;
 
-/* budget_bytes 0: 64 GiB; tick_paths 0: 8 Mi paths. */
+/* budget_bytes 0: 64 GiB; tick_paths: final-render paths per tick, 0 = tuned
+ * (each tick aimed at 12 ms from the scenes' measured cost per path). */
 int rts_create(int32_t device, uint64_t budget_bytes, uint32_t tick_paths, rts_server** out);
 /* Uploads a compiled scene (kept resident); its render settings supply the
  * tone map, exposure, roulette depth and light sampling of every request. */

@@ -821,6 +821,14 @@ struct Segment {
   uint32_t job, p0, n_pix, s0, k;
 };
 
+// A job of at least this many paths fills the GPU by itself: it gains
+// nothing from sharing launches and runs faster through the single-render
+// kernels, whose scene and parameters are kernel arguments (the batch
+// kernels read them per virtual block from the job table, which also costs
+// them registers). Measured: one 1024x1024, 64-spp torus render in 64 Mi
+// paths, 55 ms alone vs 65 ms as single-job batches.
+constexpr uint64_t kBatchSoloPaths = 1ull << 22;
+
 bool batchable(const rtg_job& j) {
   return j.settings.precision == RTG_PRECISION_F32 && j.settings.integrator == RTG_INTEGRATOR_PATH &&
          j.settings.max_depth <= kBatchDepth;
@@ -985,11 +993,15 @@ rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jo
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<rtg_stats> acc_stats(n_jobs, rtg_stats{});
-  // Jobs outside the batch kernels' scope, through the single-render path.
+  // Jobs outside the batch kernels' scope, and jobs large enough to fill
+  // the GPU alone, through the single-render path.
   uint64_t batch_paths = 0;
+  std::vector<uint8_t> in_batch(n_jobs, 0);
   for (uint32_t i = 0; i < n_jobs; ++i) {
-    if (batchable(jobs[i])) {
-      batch_paths += (uint64_t)(jobs[i].s1 - jobs[i].s0) * (p1[i] - p0[i]);
+    const uint64_t paths = (uint64_t)(jobs[i].s1 - jobs[i].s0) * (p1[i] - p0[i]);
+    if (batchable(jobs[i]) && paths < kBatchSoloPaths) {
+      in_batch[i] = 1;
+      batch_paths += paths;
       continue;
     }
     rtg_stats st{};
@@ -1006,7 +1018,7 @@ rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jo
     // packed in job order into groups of at most `cap` paths.
     std::vector<Segment> segs;
     for (uint32_t i = 0; i < n_jobs; ++i) {
-      if (!batchable(jobs[i])) continue;
+      if (!in_batch[i]) continue;
       const uint32_t k_total = jobs[i].s1 - jobs[i].s0;
       for (uint32_t sa = jobs[i].s0; sa < jobs[i].s1;) {
         const uint32_t k = std::min(k_total - (sa - jobs[i].s0), cap);

@@ -59,7 +59,7 @@ void usage() {
                "                         {\"job\":N,\"scene\":{...},overrides...} or {\"cancel\":N}; streams\n"
                "                         FRAME/DONE/ERROR lines tagged with the job id\n"
                "  --supersede            (serve) a new job cancels the running ones\n"
-               "  --tick-paths N         (serve) paths of large jobs per batched pass, default 8388608\n"
+               "  --tick-paths N         (serve) paths of large jobs per batched pass (default: tuned per tick)\n"
                "  --cache-mb N           (serve) resident compiled scenes, default 8192\n";
 }
 

@@ -35,6 +35,7 @@
 #include "json.hpp"
 #include "renderer.hpp"
 #include "scene.hpp"
+#include "tick_tuner.hpp"
 
 namespace rtm {
 
@@ -100,7 +101,7 @@ uint32_t pass_samples(uint32_t done, uint32_t spp, uint32_t max_pass) {
 class Service {
  public:
   Service(int device, bool supersede, uint32_t tick_paths, uint64_t cache_bytes, bool quiet)
-      : supersede_(supersede), tick_paths_(tick_paths), cache_bytes_(cache_bytes), quiet_(quiet) {
+      : supersede_(supersede), tuner_(tick_paths, kTickTargetMs), cache_bytes_(cache_bytes), quiet_(quiet) {
     rtg_check(rtg_open(device, &ctx_), nullptr, "rtg_open");
   }
   ~Service() {
@@ -141,16 +142,19 @@ class Service {
     std::vector<rtg_job> batch;
     std::vector<Job*> owner;
     std::vector<uint32_t> pixels;
-    int64_t budget = tick_paths_;
+    tuner_.begin();
     for (int kind = 0; kind < 2; ++kind)  // previews first, whole
       for (Job& j : jobs_) {
         if (j.preview != (kind == 0)) continue;
         const uint32_t n_pix = (uint32_t)j.settings.width * (uint32_t)j.settings.height;
         uint32_t take = n_pix - j.cursor;
         if (!j.preview) {
-          if (budget <= 0) continue;
-          take = (uint32_t)std::min<int64_t>(take, std::max<int64_t>(1, budget / j.k));
-          budget -= (int64_t)take * j.k;
+          const uint64_t allow = tuner_.final_paths(tune_key(j));
+          if (!allow) continue;
+          take = (uint32_t)std::min<uint64_t>(take, std::max<uint64_t>(1, allow / j.k));
+          tuner_.charge_final(tune_key(j), (uint64_t)take * j.k);
+        } else {
+          tuner_.charge_preview(tune_key(j), (uint64_t)take * j.k);
         }
         rtg_job b{};
         b.scene = j.view;
@@ -165,7 +169,13 @@ class Service {
         pixels.push_back(take);
       }
     std::vector<rtg_stats> st(batch.size());
+    const auto tick0 = Clock::now();
     const rtg_status s = rtg_render_batch(ctx_, batch.data(), (uint32_t)batch.size(), st.data());
+    if (s == RTG_OK) {
+      for (size_t q = 0; q < batch.size(); ++q)
+        tuner_.add_job(tune_key(*owner[q]), (uint64_t)pixels[q] * owner[q]->k, st[q].rays + st[q].shadow_rays);
+      tuner_.end(ms_between(tick0, Clock::now()));
+    }
     if (s != RTG_OK) {
       const std::string msg = rtg_last_error(ctx_);
       for (Job* j : owner) {
@@ -294,6 +304,9 @@ class Service {
       if (victim == cache_.end()) return;
       total -= victim->second.bytes;
       rtg_scene_release(victim->second.dev);
+      const char* key = reinterpret_cast<const char*>(&victim->second);
+      tuner_.forget(key);
+      tuner_.forget(key + 1);
       cache_.erase(victim);
     }
   }
@@ -351,7 +364,12 @@ class Service {
 
   rtg_context* ctx_ = nullptr;
   bool supersede_;
-  uint32_t tick_paths_;
+  TickTuner tuner_;
+  // Tuner key of a job: its resident scene and kind (previews and finals
+  // differ in depth and size, so their costs per path are kept apart).
+  const void* tune_key(const Job& j) const {
+    return reinterpret_cast<const char*>(&cache_.at(j.key)) + (j.preview ? 0 : 1);
+  }
   uint64_t cache_bytes_;
   bool quiet_;
   std::list<Job> jobs_;
@@ -364,8 +382,7 @@ class Service {
 int serve_main(int device, bool supersede, uint32_t tick_paths, uint64_t cache_bytes, bool quiet) {
   std::unique_ptr<Service> svc;
   try {
-    svc = std::make_unique<Service>(device, supersede, tick_paths ? tick_paths : (8u << 20),
-                                    cache_bytes ? cache_bytes : (8ull << 30), quiet);
+    svc = std::make_unique<Service>(device, supersede, tick_paths, cache_bytes ? cache_bytes : (8ull << 30), quiet);
   } catch (const std::exception& e) {
     std::cout << "ERROR {\"job\":-1,\"message\":" << jstr(e.what()) << "}\n" << std::flush;
     return 2;

@@ -42,7 +42,7 @@ uint32_t pass_samples(uint32_t done, uint32_t spp, uint32_t max_pass) {
 }  // namespace
 
 Server::Server(int device, uint64_t budget_bytes, uint32_t tick_paths)
-    : budget_(budget_bytes ? budget_bytes : (64ull << 30)), tick_paths_(tick_paths ? tick_paths : (8u << 20)) {
+    : budget_(budget_bytes ? budget_bytes : (64ull << 30)), tuner_(tick_paths, kTickTargetMs) {
   rtg_check(rtg_open(device, &ctx_), nullptr, "rtg_open");
 }
 
@@ -222,11 +222,11 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, bool served, std::
       continue;
     }
     // One tick: every preview's whole pass, then finals in admission order
-    // within the tick's path budget (whole pixels, at least one).
+    // within the tick's budget (whole pixels, at least one).
     jobs.clear();
     job_pixels.clear();
     job_live.clear();
-    int64_t budget = tick_paths_;
+    tuner_.begin();
     for (int pass_kind = 0; pass_kind < 2; ++pass_kind)
       for (size_t li = 0; li < live.size(); ++li) {
         const Live& L = live[li];
@@ -235,9 +235,12 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, bool served, std::
         const uint32_t n_pix = (uint32_t)r.width * (uint32_t)r.height;
         uint32_t take = n_pix - L.cursor;
         if (r.final_kind) {
-          if (budget <= 0) continue;
-          take = std::min<uint64_t>(take, std::max<int64_t>(1, budget / L.k));
-          budget -= (int64_t)take * L.k;
+          const uint64_t allow = tuner_.final_paths(tune_key(r));
+          if (!allow) continue;
+          take = (uint32_t)std::min<uint64_t>(take, std::max<uint64_t>(1, allow / L.k));
+          tuner_.charge_final(tune_key(r), (uint64_t)take * L.k);
+        } else {
+          tuner_.charge_preview(tune_key(r), (uint64_t)take * L.k);
         }
         rtg_job j{};
         j.scene = scene_for(r);
@@ -252,8 +255,17 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, bool served, std::
         job_live.push_back(li);
       }
     js.assign(jobs.size(), rtg_stats{});
+    const auto tick0 = std::chrono::steady_clock::now();
     rtg_check(rtg_render_batch(ctx_, jobs.data(), (uint32_t)jobs.size(), js.data()), ctx_, "rtg_render_batch");
+    const double tick_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tick0).count();
     ++st.ticks;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      const ServeRequest& rq = reqs[live[job_live[q]].i];
+      tuner_.add_job(tune_key(rq), (uint64_t)job_pixels[q] * (jobs[q].s1 - jobs[q].s0),
+                     js[q].rays + js[q].shadow_rays);
+    }
+    tuner_.end(tick_ms);
     for (size_t q = 0; q < jobs.size(); ++q) {
       Live& L = live[job_live[q]];
       const ServeRequest& r = reqs[L.i];

@@ -9,8 +9,10 @@
 //   * previews (small, few samples) render all their samples in the next
 //     tick, so they never wait behind a large render;
 //   * finals advance their progressive pass schedule (1, 1, 2, 4, ... capped
-//     at max_pass, renderer.cpp:66-69) by at most `tick_paths` paths per tick,
-//     split into whole-pixel ranges; a frame is resolved when a pass ends;
+//     at max_pass, renderer.cpp:66-69) in whole-pixel ranges sized by the
+//     tick tuner (tick_tuner.hpp: each tick aimed at kTickTargetMs from the
+//     scenes' measured cost per path) or by a fixed `tick_paths` paths per
+//     tick; a frame is resolved when a pass ends;
 //   * requests are admitted first-come first-served while the accumulation
 //     buffers of admitted requests fit the memory budget.
 // A request's image is bitwise the image it gets when rendered alone: its
@@ -25,6 +27,7 @@
 
 #include "rtg.h"
 #include "scene.hpp"
+#include "tick_tuner.hpp"
 
 namespace rtm {
 
@@ -54,7 +57,8 @@ struct ServeStats {
 class Server {
  public:
   // budget_bytes: accumulation buffers of admitted requests (0: 64 GiB);
-  // tick_paths: paths of final renders per tick (0: 8 Mi).
+  // tick_paths: paths of final renders per tick (0: tuned, each tick aimed
+  // at kTickTargetMs).
   Server(int device, uint64_t budget_bytes, uint32_t tick_paths);
   ~Server();
   Server(const Server&) = delete;
@@ -87,7 +91,12 @@ class Server {
   rtg_context* ctx_ = nullptr;
   std::vector<SceneSlot> scenes_;
   uint64_t budget_;
-  uint32_t tick_paths_;
+  TickTuner tuner_;
+  // Tuner key of a request: its scene and kind (previews and finals differ
+  // in depth and size, so their costs per path are kept apart).
+  static const void* tune_key(const ServeRequest& r) {
+    return reinterpret_cast<const void*>(static_cast<uintptr_t>(r.scene) * 2 + (r.final_kind ? 1 : 0) + 1);
+  }
   std::vector<uint8_t> rgb_;
   static constexpr uint64_t kPoolItemBytes = 256ull * 256 * 28, kPoolBytes = 1ull << 30;
   std::multimap<std::pair<int, int>, rtg_accum*> pool_;

@@ -102,12 +102,14 @@ def test_pixel_ranges_of_one_buffer(ctx):
 
 
 def test_jobs_outside_the_batch_kernels(ctx):
-    """FP64 and direct/normals jobs go through the single-render path."""
+    """FP64 and direct/normals jobs, and jobs of at least 4 Mi paths, go
+    through the single-render path."""
     sp = [job_spec(scenes.load_asset("materials.json"), 20, 12, 2, precision=rtg.PRECISION_F64),
           job_spec(scenes.load_asset("materials.json"), 20, 12, 2, integrator="direct"),
           job_spec(scenes.load_asset("mirrors.json"), 16, 8, 2, integrator="normals"),
+          job_spec(scenes.load("cornell_c1.json"), 1024, 1024, 4),
           job_spec(scenes.load("cornell_c1.json"), 16, 16, 2)]
-    res, (stats,) = batched(ctx, sp, [(0, 2)])
+    res, (stats,) = batched(ctx, sp, [(0, 4)])
     for (o, cfg, spp), (s, c), st in zip(sp, res, stats):
         s1, c1, st1 = solo(ctx, o, cfg, 0, spp)
         assert np.array_equal(c, c1) and np.array_equal(s, s1)
