@@ -56,10 +56,11 @@ def render_once(rtg, ctx, dev, acc, cfg, spp, max_pass=32):
     return rays
 
 
-def cpu_baseline(scene_text, width, height, target_s=12.0):
+def cpu_baseline(scene_text, width, height, spp_full, target_s=12.0):
     """The reference renderer (oracle/_ref, built from the reference's own
     sources) on this host's cores, on a bounded sample of the workload:
-    the full image at a reduced sample count chosen for ~target_s seconds."""
+    whole renders of the full image at a sample count of at most the
+    workload's (spp_full), chosen so that the sample takes ~target_s seconds."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -81,13 +82,13 @@ def cpu_baseline(scene_text, width, height, target_s=12.0):
             _, _, st = o.render(0, spp, threads=threads)
             return st["rays"] + st["shadow_rays"], time.perf_counter() - t
     rays1, s1 = run(1)
-    spp = int(max(1, min(64, round(target_s / max(s1, 1e-3)))))
+    spp = int(max(1, min(spp_full, round(target_s / max(s1, 1e-3)))))
     rays, secs, reps = 0, 0.0, 0
     while secs < target_s * 0.8 and reps < 8:  # whole renders until ~10 s of CPU work
         r, t = run(spp) if spp > 1 else (rays1, s1)
         rays, secs, reps = rays + r, secs + t, reps + 1
     return {"value": rays / secs / 1e9, "unit": "G rays/s", "cores": threads, "kind": kind,
-            "sample": f"{reps} x {width}x{height} at {spp} spp (of 64), progressive passes, "
+            "sample": f"{reps} x {width}x{height} at {spp} spp (of {spp_full}), progressive passes, "
                       f"{rays} rays in {secs:.2f} s"}
 
 
@@ -317,7 +318,7 @@ def main():
                                                      tick_paths=args.tick_paths)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(text, cfg.width, cfg.height)
+        cpu = cpu_baseline(text, cfg.width, cfg.height, spp)
     line = {
         "metric": "G rays/s + % HBM roofline; served p50/p99 latency vs plain GPU loop",
         "value": value, "unit": "G rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
