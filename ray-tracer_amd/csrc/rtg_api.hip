@@ -1099,7 +1099,7 @@ rtg_status rtg_trace(rtg_context* ctx, const rtg_scene* scene, const rtg_buffer*
   if (n) {
     if (opts->precision == RTG_PRECISION_F64)
       run_trace64(scene->dev, static_cast<const double*>(rays->ptr), n, opts->mode, opts->use_bvh,
-                  static_cast<rtg_hit*>(hits->ptr), ctx->stats, ctx->stream, hooks_of(ctx));
+                  static_cast<rtg_hit*>(hits->ptr), ctx->stats, scene->stack, ctx->stream, hooks_of(ctx));
     else
       run_trace32(scene->dev, static_cast<const double*>(rays->ptr), n, opts->mode, opts->use_bvh,
                   static_cast<rtg_hit*>(hits->ptr), ctx->stats, scene->stack, ctx->stream, hooks_of(ctx));

@@ -44,27 +44,52 @@ __global__ void k_resolve(const double* sum, const uint32_t* count, int n, int o
 }
 }  // namespace
 
-void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t, hipStream_t s,
-               const LaunchHooks* h) {
+namespace {
+template <int STACK>
+void chunk_impl(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, hipStream_t s,
+                const LaunchHooks* h) {
   const uint32_t g = grid_of(P.n_paths);
   if (P.integrator != RTG_INTEGRATOR_PATH) {
-    RTG_LAUNCH(h, RTG_K_DIRECT, (k_direct<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P);
+    RTG_LAUNCH(h, RTG_K_DIRECT, (k_direct<double, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P);
   } else {
     RTG_LAUNCH(h, RTG_K_GENERATE, k_generate<double>, dim3(grid_of(generate_threads(P))), dim3(kBlock), 0, s, S, B, P);
     const bool nee = P.light_sampling && S.n_lights > 0;
     for (int b = 0; b < P.max_depth; ++b) {
-      RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend<double, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
       RTG_LAUNCH(h, RTG_K_SHADE, k_shade<double>, dim3(g), dim3(kBlock), 0, s, S, B, P, b);
-      if (nee) RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<double, 0>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      if (nee) RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<double, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
     }
   }
   RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_accumulate<double>, dim3(accumulate_blocks<double>(P)), dim3(kBlock), 0, s, B, P);
 }
 
+template <int STACK>
+void trace_impl(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,
+                unsigned long long* stats, hipStream_t s, const LaunchHooks* h) {
+  RTG_LAUNCH(h, RTG_K_TRACE, (k_trace<double, STACK>), dim3(grid_of(n)), dim3(kBlock), 0, s, S, rays, n, mode,
+             use_bvh, hits, stats);
+}
+}  // namespace
+
+// The reference's traversal pushes one sibling per inner node on the path,
+// so its stack, kept in LDS, needs the BVH depth in entries: the FP32 stack
+// classes serve it.
+void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
+               const LaunchHooks* h) {
+  switch (stack_class_of(stack)) {
+    case 0: chunk_impl<19>(S, B, P, s, h); break;
+    case 1: chunk_impl<31>(S, B, P, s, h); break;
+    default: chunk_impl<63>(S, B, P, s, h); break;
+  }
+}
+
 void run_trace64(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,
-                 unsigned long long* stats, hipStream_t s, const LaunchHooks* h) {
-  RTG_LAUNCH(h, RTG_K_TRACE, (k_trace<double, 0>), dim3(grid_of(n)), dim3(kBlock), 0, s, S, rays, n, mode, use_bvh,
-             hits, stats);
+                 unsigned long long* stats, uint32_t stack, hipStream_t s, const LaunchHooks* h) {
+  switch (stack_class_of(stack)) {
+    case 0: trace_impl<19>(S, rays, n, mode, use_bvh, hits, stats, s, h); break;
+    case 1: trace_impl<31>(S, rays, n, mode, use_bvh, hits, stats, s, h); break;
+    default: trace_impl<63>(S, rays, n, mode, use_bvh, hits, stats, s, h); break;
+  }
 }
 
 void run_resolve(const double* sum, const uint32_t* count, int w, int h, int op, double exposure, uint8_t* rgb,

@@ -180,12 +180,13 @@ struct ModeOps;
 template <>
 struct ModeOps<double> {
   using R = PcgDraws;
-  __device__ static Hit64 closest(const DevScene& S, bool bvh, V3<double> o, V3<double> d, LdsStack, Counters& c) {
-    return closest64(S, bvh, o, d, 1e-6, __builtin_inf(), c);
+  __device__ static Hit64 closest(const DevScene& S, bool bvh, V3<double> o, V3<double> d, LdsStack stk,
+                                 Counters& c) {
+    return closest64(S, bvh, o, d, 1e-6, __builtin_inf(), stk, c);
   }
-  __device__ static bool occluded(const DevScene& S, bool bvh, V3<double> o, V3<double> d, double tmax, LdsStack,
-                                  Counters& c) {
-    return any64(S, bvh, o, d, 1e-6, tmax, c);
+  __device__ static bool occluded(const DevScene& S, bool bvh, V3<double> o, V3<double> d, double tmax,
+                                  LdsStack stk, Counters& c) {
+    return any64(S, bvh, o, d, 1e-6, tmax, stk, c);
   }
   __device__ static void finalize(const DevScene& S, V3<double> o, V3<double> d, double t, double b1, double b2,
                                   uint32_t prim, int plane, HitRec<double>& rec) {
@@ -751,7 +752,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const double* __re
     if constexpr (sizeof(T) == 8) {
       const V3<double> o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
       if (mode == RTG_TRACE_CLOSEST) {
-        const Hit64 h = closest64(S, use_bvh != 0, o, d, r[6], r[7], c);
+        const Hit64 h = closest64(S, use_bvh != 0, o, d, r[6], r[7], stk, c);
         if (h.prim != 0xFFFFFFFFu || h.plane >= 0) {
           HitRec<double> rec;
           finalize64(S, o, d, h.t, h.b1, h.b2, h.prim, h.plane, rec);
@@ -760,7 +761,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const double* __re
           out.flags = 1u | (h.prim == 0xFFFFFFFFu ? 2u : 0u) | (rec.front_face ? 4u : 0u);
         }
       } else {
-        out.flags = any64(S, use_bvh != 0, o, d, 1e-6, r[7], c) ? 1u : 0u;
+        out.flags = any64(S, use_bvh != 0, o, d, 1e-6, r[7], stk, c) ? 1u : 0u;
       }
     } else {
       const Ray32 ray = make_ray32((float)r[0], (float)r[1], (float)r[2], (float)r[3], (float)r[4], (float)r[5]);

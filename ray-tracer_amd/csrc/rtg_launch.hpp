@@ -52,7 +52,7 @@ void batch_occupancy(uint32_t stack, uint32_t* trace, uint32_t* shade, uint32_t*
 void run_trace32(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,
                  unsigned long long* stats, uint32_t stack, hipStream_t s, const LaunchHooks* hooks);
 void run_trace64(const DevScene& S, const double* rays, uint64_t n, int mode, int use_bvh, rtg_hit* hits,
-                 unsigned long long* stats, hipStream_t s, const LaunchHooks* hooks);
+                 unsigned long long* stats, uint32_t stack, hipStream_t s, const LaunchHooks* hooks);
 
 // Tone map + sRGB8 (tonemap.cpp:14-53), FP64 like the reference.
 void run_resolve(const double* sum, const uint32_t* count, int w, int h, int op, double exposure, uint8_t* rgb,

@@ -26,6 +26,19 @@ struct Counters {
   uint32_t visits, tests;
 };
 
+// LDS stack: entry k of this lane lives at ref[k * kStackStride] (one
+// column per thread of a traversal workgroup, which has kStackStride threads).
+// The FP32 stack holds child refs only (4 bytes per entry): half the LDS of
+// a {ref, entry distance} stack doubles the resident waves, which is worth
+// more than culling far entries on pop (measured on the torus workload:
+// closest-hit 40.4 -> 29.0 ms per render for 5 % more node visits). A far
+// child popped after a closer hit is visited again; its boxes lie inside the
+// pushed box, so it cannot change the result.
+constexpr uint32_t kStackStride = 128;
+struct LdsStack {
+  uint32_t* ref;
+};
+
 // =========================== FP64 (exact) ===============================
 
 __device__ __forceinline__ V3<double> d3(rtg_vec3 v) { return V3<double>{v.x, v.y, v.z}; }
@@ -132,7 +145,7 @@ struct Hit64 {
 
 // Scene::intersect (scene.cpp:176-198) without finalize.
 __device__ inline Hit64 closest64(const DevScene& S, bool use_bvh, V3<double> o, V3<double> d, double tmin,
-                                  double tmax, Counters& c) {
+                                  double tmax, LdsStack stk, Counters& c) {
   Hit64 h;
   h.plane = planes64(S, o, d, tmin, tmax);
   h.t = tmax;
@@ -155,8 +168,7 @@ __device__ inline Hit64 closest64(const DevScene& S, bool use_bvh, V3<double> o,
   if (S.n_nodes == 0) return h;
   const V3<double> inv{1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
   const bool neg[3] = {inv.x < 0, inv.y < 0, inv.z < 0};
-  uint32_t stack[64];
-  int sp = 0;
+  uint32_t sp = 0;  // the stack holds one pushed sibling per ancestor: at most the BVH depth
   uint32_t idx = 0;
   double tm = tmax;
   for (;;) {
@@ -178,24 +190,24 @@ __device__ inline Hit64 closest64(const DevScene& S, bool use_bvh, V3<double> o,
         }
       } else {
         if (neg[n.axis]) {
-          stack[sp++] = idx + 1;
+          stk.ref[sp++ * kStackStride] = idx + 1;
           idx = n.offset;
         } else {
-          stack[sp++] = n.offset;
+          stk.ref[sp++ * kStackStride] = n.offset;
           idx = idx + 1;
         }
         continue;
       }
     }
     if (sp == 0) break;
-    idx = stack[--sp];
+    idx = stk.ref[--sp * kStackStride];
   }
   return h;
 }
 
 // Scene::occluded (scene.cpp:217-234); t_min is the caller's (1e-6 there).
 __device__ inline bool any64(const DevScene& S, bool use_bvh, V3<double> o, V3<double> d, double tmin, double tmax,
-                             Counters& c) {
+                             LdsStack stk, Counters& c) {
   for (uint32_t i = 0; i < S.n_planes; ++i) {
     const rtg_plane& pl = S.planes[i];
     const V3<double> n = d3(pl.normal);
@@ -214,8 +226,7 @@ __device__ inline bool any64(const DevScene& S, bool use_bvh, V3<double> o, V3<d
   }
   if (S.n_nodes == 0) return false;
   const V3<double> inv{1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
-  uint32_t stack[64];
-  int sp = 0;
+  uint32_t sp = 0;
   uint32_t idx = 0;
   for (;;) {
     const rtg_bvh_node n = S.nodes[idx];
@@ -227,13 +238,13 @@ __device__ inline bool any64(const DevScene& S, bool use_bvh, V3<double> o, V3<d
           if (prim_hit64(S.prims[S.prim_indices[n.offset + i]], o, d, tmin, tmax, th, b1, b2)) return true;
         }
       } else {
-        stack[sp++] = n.offset;
+        stk.ref[sp++ * kStackStride] = n.offset;
         idx = idx + 1;
         continue;
       }
     }
     if (sp == 0) break;
-    idx = stack[--sp];
+    idx = stk.ref[--sp * kStackStride];
   }
   return false;
 }
@@ -375,18 +386,6 @@ struct Hit32 {
   int plane;
 };
 
-// LDS stack: entry k of this lane lives at ref[k * kStackStride] (one
-// column per thread of a traversal workgroup, which has kStackStride threads).
-// The FP32 stack holds child refs only (4 bytes per entry): half the LDS of
-// a {ref, entry distance} stack doubles the resident waves, which is worth
-// more than culling far entries on pop (measured on the torus workload:
-// closest-hit 40.4 -> 29.0 ms per render for 5 % more node visits). A far
-// child popped after a closer hit is visited again; its boxes lie inside the
-// pushed box, so it cannot change the result.
-constexpr uint32_t kStackStride = 128;
-struct LdsStack {
-  uint32_t* ref;
-};
 
 // "No node": never a valid child reference (a leaf's first slot plus its
 // count never exceeds 2^25), and it has the leaf bit, so inner() is false.
