@@ -92,6 +92,26 @@ def cpu_baseline(scene_text, width, height, spp_full, target_s=12.0):
                       f"{rays} rays in {secs:.2f} s"}
 
 
+def extend_roofline(prof, traffic):
+    """Roofline of the closest-hit kernel from a profile: algorithmic bytes per
+    launch (node boxes, triangle tests, ray in + hit out) over its average
+    launch duration (HIP events around every launch on its stream)."""
+    ext = prof["extend"]
+    n_launch = max(1, ext["launches"])
+    bytes_total = (NODE_B * prof["closest_visits"] + TRI_B * prof["closest_tests"] +
+                   (RAY_IN_B + HIT_OUT_B) * prof["closest_rays"])
+    per_launch = bytes_total / n_launch
+    avg_ms = ext["ms"] / n_launch
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return {"kernel": "k_extend (FP32 closest-hit traversal)", "bound": "hbm", "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches": ext["launches"],
+            "node_visits_per_ray": prof["closest_visits"] / max(1, prof["closest_rays"]),
+            "tests_per_ray": prof["closest_tests"] / max(1, prof["closest_rays"]),
+            "shadow_node_visits_per_ray": prof["shadow_visits"] / max(1, prof["shadow_rays"]),
+            "shadow_tests_per_ray": prof["shadow_tests"] / max(1, prof["shadow_rays"])}
+
+
 def pass_schedule(spp, max_pass=32):
     """Progressive passes (renderer.cpp:66-69): 1, 1, 2, 4, ... capped."""
     out, done = [], 0
@@ -289,27 +309,25 @@ def main():
     prof = rtg.get_profile(ctx).as_dict()
     rtg.set_profiling(ctx, False)
     value = rays / secs / 1e9
-
-    ext = prof["extend"]
-    n_launch = max(1, ext["launches"])
-    bytes_total = (NODE_B * prof["closest_visits"] + TRI_B * prof["closest_tests"] +
-                   (RAY_IN_B + HIT_OUT_B) * prof["closest_rays"])
-    per_launch = bytes_total / n_launch
-    avg_ms = ext["ms"] / n_launch
-    achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_extend_{args.workload}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-    roofline = {"kernel": "k_extend (FP32 closest-hit traversal)", "bound": "hbm", "achieved": achieved,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches": ext["launches"],
-                "node_visits_per_ray": prof["closest_visits"] / max(1, prof["closest_rays"]),
-                "tests_per_ray": prof["closest_tests"] / max(1, prof["closest_rays"]),
-                "shadow_node_visits_per_ray": prof["shadow_visits"] / max(1, prof["shadow_rays"]),
-                "shadow_tests_per_ray": prof["shadow_tests"] / max(1, prof["shadow_rays"])}
+    roofline = extend_roofline(prof, traffic)
+    roofline["concurrent"] = ("the any-hit kernel of the previous bounce runs beside each launch on a second "
+                              "stream, so a launch's duration includes time shared with it")
     kernel_ms = {k: round(v["ms"] / args.steps, 3) for k, v in prof.items() if isinstance(v, dict)}
+    # The same kernel with the any-hit kernels on the main stream (not timed
+    # for `value`): the traversal kernel's own rate.
+    os.environ["RTG_OVERLAP"] = "0"
+    rtg.set_profiling(ctx, True)
+    for _ in range(2):
+        render_once(rtg, ctx, dev, acc, cfg, spp)
+    ctx.synchronize()
+    roofline_serial = extend_roofline(rtg.get_profile(ctx).as_dict(), traffic)
+    rtg.set_profiling(ctx, False)
+    del os.environ["RTG_OVERLAP"]
     dev.release()
     acc.release()
     ctx.close()
@@ -333,6 +351,7 @@ def main():
                    "shadow_rays_per_step": (rays - prof["closest_rays"]) // args.steps,
                    "kernel_ms_per_step": kernel_ms},
         "roofline": roofline,
+        "roofline_serial": roofline_serial,
         "cpu_baseline": cpu,
         "batched": batch,
         "served": served,

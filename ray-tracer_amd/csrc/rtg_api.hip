@@ -36,6 +36,7 @@ struct rtg_context {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  SideStream side{};  // default priority, like the main stream
   std::string err;
   unsigned long long* stats = nullptr;
   int n_cu = 0;
@@ -56,6 +57,7 @@ struct rtg_context {
   // buffers, so they are dropped whenever those are reallocated.
   struct BatchGraph {
     int depth, cls;
+    bool side;  // captured with the any-hit kernels on the side stream
     hipGraphExec_t exec;
   };
   std::vector<BatchGraph> batch_graphs;
@@ -203,7 +205,7 @@ rtg_status validate_desc(rtg_context* ctx, const rtg_scene_desc* d) {
   return RTG_OK;
 }
 
-void profile_mark(void* user, int kind, bool begin) {
+void profile_mark(void* user, int kind, bool begin, hipStream_t stream) {
   auto* ctx = static_cast<rtg_context*>(user);
   if (ctx->events_used == ctx->events.size()) {
     hipEvent_t e;
@@ -214,7 +216,7 @@ void profile_mark(void* user, int kind, bool begin) {
     if (ctx->event_kind.size() < ctx->events_used + 1) ctx->event_kind.resize(ctx->events_used + 1);
     ctx->event_kind[ctx->events_used] = kind;
   }
-  hipEventRecord(ctx->events[ctx->events_used++], ctx->stream);
+  hipEventRecord(ctx->events[ctx->events_used++], stream);
 }
 
 // After the stream has drained: add every begin/end pair to the totals.
@@ -235,6 +237,13 @@ void profile_resolve(rtg_context* ctx, const unsigned long long* h) {
   ctx->profile.shadow_rays += h[kStShadow];
   ctx->profile.shadow_visits += h[kStSVisits];
   ctx->profile.shadow_tests += h[kStSTests];
+}
+
+// The side stream for overlapping a pass's any-hit kernels, unless
+// RTG_OVERLAP=0 (everything on the main stream).
+const SideStream* side_of(rtg_context* ctx) {
+  const char* e = std::getenv("RTG_OVERLAP");
+  return (e && e[0] == '0') ? nullptr : &ctx->side;
 }
 
 const LaunchHooks* hooks_of(rtg_context* ctx) {
@@ -277,6 +286,9 @@ rtg_status rtg_open(int device, rtg_context** out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side.stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.shaded, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.shadowed, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&c->stats, sizeof(unsigned long long) * kStatSlots * kStatStride);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) {
@@ -302,6 +314,12 @@ rtg_status rtg_close(rtg_context* ctx) {
   if (ctx->rgb_dev) hipFree(ctx->rgb_dev);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  if (ctx->side.stream) {
+    hipStreamSynchronize(ctx->side.stream);
+    hipStreamDestroy(ctx->side.stream);
+  }
+  if (ctx->side.shaded) hipEventDestroy(ctx->side.shaded);
+  if (ctx->side.shadowed) hipEventDestroy(ctx->side.shadowed);
   for (hipEvent_t e : ctx->events) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -800,7 +818,7 @@ rtg_status render_range(rtg_context* ctx, const rtg_scene* scene, const rtg_rend
       P.counts = f64 ? ctx->pb64.counts : B32.counts;
       P.stats = ctx->stats;
       if (f64) run_chunk(scene->dev, ctx->pb64, P, scene->stack, ctx->stream, hooks_of(ctx));
-      else run_chunk(scene->dev, B32, P, scene->stack, ctx->stream, hooks_of(ctx));
+      else run_chunk(scene->dev, B32, P, scene->stack, ctx->stream, side_of(ctx), hooks_of(ctx));
       HIP_TRY(ctx, hipGetLastError());
       if (cancel) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // poll between chunks
     }
@@ -912,6 +930,7 @@ rtg_status run_group(rtg_context* ctx, const rtg_job* jobs, const Segment* seg, 
   // replay with the arguments they were captured with) and the surplus
   // workgroups find no block and exit.
   const bool graph = !ctx->profiling && batch_graphs_enabled();
+  const SideStream* side = side_of(ctx);
   BatchGrids g;
   g.generate = std::max(1u, graph ? occ[2] * cu : std::min(gen_blocks, occ[2] * cu));
   g.trace = std::max(1u, graph ? occ[0] * cu : std::min(path_blocks, occ[0] * cu));
@@ -920,11 +939,11 @@ rtg_status run_group(rtg_context* ctx, const rtg_job* jobs, const Segment* seg, 
   if (graph) {
     hipGraphExec_t exec = nullptr;
     for (const auto& e : ctx->batch_graphs)
-      if (e.depth == depth && e.cls == cls) exec = e.exec;
+      if (e.depth == depth && e.cls == cls && e.side == (side != nullptr)) exec = e.exec;
     if (!exec) {
       hipGraph_t gr = nullptr;
       HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      run_batch(ctx->batch_jobs, B, L, depth, stack, g, ctx->stream, nullptr);
+      run_batch(ctx->batch_jobs, B, L, depth, stack, g, ctx->stream, side, nullptr);
       const hipError_t le = hipGetLastError();
       const hipError_t ce = hipStreamEndCapture(ctx->stream, &gr);
       hipError_t ie = hipSuccess;
@@ -933,11 +952,11 @@ rtg_status run_group(rtg_context* ctx, const rtg_job* jobs, const Segment* seg, 
       for (hipError_t e : {le, ce, ie})
         if (e != hipSuccess)
           return fail(ctx, RTG_ERR_HIP, std::string("rtg_render_batch: capturing a batch pass: ") + hipGetErrorString(e));
-      ctx->batch_graphs.push_back({depth, cls, exec});
+      ctx->batch_graphs.push_back({depth, cls, side != nullptr, exec});
     }
     HIP_TRY(ctx, hipGraphLaunch(exec, ctx->stream));
   } else {
-    run_batch(ctx->batch_jobs, B, L, depth, stack, g, ctx->stream, hooks_of(ctx));
+    run_batch(ctx->batch_jobs, B, L, depth, stack, g, ctx->stream, side, hooks_of(ctx));
     HIP_TRY(ctx, hipGetLastError());
   }
   static thread_local std::vector<unsigned long long> slots;

@@ -11,33 +11,68 @@ inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBloc
 
 template <int STACK>
 void chunk_impl(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P, hipStream_t s,
-                const LaunchHooks* h) {
+                const SideStream* side, const LaunchHooks* h) {
   const uint32_t g = grid_of(P.n_paths);
   if (P.integrator != RTG_INTEGRATOR_PATH) {
     RTG_LAUNCH(h, RTG_K_DIRECT, (k_direct<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P);
   } else {
     RTG_LAUNCH(h, RTG_K_GENERATE, k_generate<float>, dim3(grid_of(generate_threads(P))), dim3(kBlock), 0, s, S, B, P);
     const bool nee = P.light_sampling && S.n_lights > 0;
+    // With a side stream, the any-hit kernel of bounce b runs beside the
+    // closest-hit kernel of bounce b + 1: it reads the shadow queue of
+    // bounce b and adds to the radiance, while extend reads the next active
+    // queue and writes hit records. The shade kernel of bounce b + 1 (which
+    // adds to the radiance and rewrites the shadow queue) and the
+    // accumulation wait for it, so every path's additions keep their order.
+    bool pending = false;
     for (int b = 0; b < P.max_depth; ++b) {
       RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      if (pending) hipStreamWaitEvent(s, side->shadowed, 0);
+      pending = false;
       RTG_LAUNCH(h, RTG_K_SHADE, k_shade<float>, dim3(g), dim3(kBlock), 0, s, S, B, P, b);
-      if (nee) RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      if (!nee) continue;
+      if (side) {
+        hipEventRecord(side->shaded, s);
+        hipStreamWaitEvent(side->stream, side->shaded, 0);
+        RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<float, STACK>), dim3(g), dim3(kBlock), 0, side->stream, S, B, P, b);
+        hipEventRecord(side->shadowed, side->stream);
+        pending = true;
+      } else {
+        RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      }
     }
+    if (pending) hipStreamWaitEvent(s, side->shadowed, 0);
   }
   RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_accumulate<float>, dim3(accumulate_blocks<float>(P)), dim3(kBlock), 0, s, B, P);
 }
 
 template <int STACK>
 void batch_impl(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPlans& L, int depth, const BatchGrids& g,
-                hipStream_t s, const LaunchHooks* h) {
+                hipStream_t s, const SideStream* side, const LaunchHooks* h) {
   RTG_LAUNCH(h, RTG_K_GENERATE, k_batch_generate<float>, dim3(g.generate), dim3(kBlock), 0, s, jobs, B, L);
   RTG_LAUNCH(h, RTG_K_PLAN, k_batch_plan<float>, dim3(1), dim3(kMaxBatchJobs), 0, s, jobs, L, -1);
+  // As in a single render, the any-hit kernel of bounce b may run beside the
+  // closest-hit kernel of bounce b + 1; the shade and planning kernels of
+  // bounce b + 1 (the planner rewrites the shadow plan) wait for it.
+  bool pending = false;
   for (int b = 0; b < depth; ++b) {
     RTG_LAUNCH(h, RTG_K_EXTEND, (k_batch_extend<float, STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
+    if (pending) hipStreamWaitEvent(s, side->shadowed, 0);
+    pending = false;
     RTG_LAUNCH(h, RTG_K_SHADE, k_batch_shade<float>, dim3(g.shade), dim3(kBlock), 0, s, jobs, B, L, b);
     RTG_LAUNCH(h, RTG_K_PLAN, k_batch_plan<float>, dim3(1), dim3(kMaxBatchJobs), 0, s, jobs, L, b);
-    RTG_LAUNCH(h, RTG_K_SHADOW, (k_batch_shadow<float, STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
+    if (side) {
+      hipEventRecord(side->shaded, s);
+      hipStreamWaitEvent(side->stream, side->shaded, 0);
+      RTG_LAUNCH(h, RTG_K_SHADOW, (k_batch_shadow<float, STACK>), dim3(g.trace), dim3(kBlock), 0, side->stream, jobs,
+                 B, L, b);
+      hipEventRecord(side->shadowed, side->stream);
+      pending = true;
+    } else {
+      RTG_LAUNCH(h, RTG_K_SHADOW, (k_batch_shadow<float, STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
+    }
   }
+  if (pending) hipStreamWaitEvent(s, side->shadowed, 0);
   RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_batch_accumulate<float>, dim3(g.accumulate), dim3(kBlock), 0, s, jobs, B, L);
 }
 
@@ -59,20 +94,20 @@ void trace_impl(const DevScene& S, const double* rays, uint64_t n, int mode, int
 }  // namespace
 
 void run_chunk(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
-               const LaunchHooks* h) {
+               const SideStream* side, const LaunchHooks* h) {
   switch (stack_class_of(stack)) {
-    case 0: chunk_impl<19>(S, B, P, s, h); break;
-    case 1: chunk_impl<31>(S, B, P, s, h); break;
-    default: chunk_impl<63>(S, B, P, s, h); break;
+    case 0: chunk_impl<19>(S, B, P, s, side, h); break;
+    case 1: chunk_impl<31>(S, B, P, s, side, h); break;
+    default: chunk_impl<63>(S, B, P, s, side, h); break;
   }
 }
 
 void run_batch(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPlans& L, int depth, uint32_t stack,
-               const BatchGrids& g, hipStream_t s, const LaunchHooks* h) {
+               const BatchGrids& g, hipStream_t s, const SideStream* side, const LaunchHooks* h) {
   switch (stack_class_of(stack)) {
-    case 0: batch_impl<19>(jobs, B, L, depth, g, s, h); break;
-    case 1: batch_impl<31>(jobs, B, L, depth, g, s, h); break;
-    default: batch_impl<63>(jobs, B, L, depth, g, s, h); break;
+    case 0: batch_impl<19>(jobs, B, L, depth, g, s, side, h); break;
+    case 1: batch_impl<31>(jobs, B, L, depth, g, s, side, h); break;
+    default: batch_impl<63>(jobs, B, L, depth, g, s, side, h); break;
   }
 }
 

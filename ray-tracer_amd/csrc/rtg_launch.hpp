@@ -19,22 +19,32 @@ inline int stack_class_of(uint32_t stack) {
 }
 
 // Optional per-launch hook (profiling): called before and after every
-// kernel launch with the RTG_K_* kind.
+// kernel launch with the RTG_K_* kind and the stream it is launched on.
 struct LaunchHooks {
   void* user;
-  void (*mark)(void* user, int kind, bool begin);
+  void (*mark)(void* user, int kind, bool begin, hipStream_t stream);
 };
-#define RTG_LAUNCH(hooks, kind, ...)                                \
-  do {                                                              \
-    if (hooks) (hooks)->mark((hooks)->user, (kind), true);          \
-    hipLaunchKernelGGL(__VA_ARGS__);                                \
-    if (hooks) (hooks)->mark((hooks)->user, (kind), false);         \
+#define RTG_LAUNCH(hooks, kind, kernel, grid, block, shmem, stream, ...)              \
+  do {                                                                                \
+    if (hooks) (hooks)->mark((hooks)->user, (kind), true, (stream));                  \
+    hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);              \
+    if (hooks) (hooks)->mark((hooks)->user, (kind), false, (stream));                 \
   } while (0)
+
+// A second stream for work of a pass that may overlap the main stream's:
+// the any-hit kernel of bounce b runs beside the closest-hit kernel of
+// bounce b + 1 (they touch disjoint path state; the shade kernel of bounce
+// b + 1 waits for both). Null: everything on the main stream.
+struct SideStream {
+  hipStream_t stream;
+  hipEvent_t shaded;    // main stream: shade of the current bounce done
+  hipEvent_t shadowed;  // side stream: any-hit of the current bounce done
+};
 
 // Runs one chunk of a render pass: generate, max_depth x (extend, shade,
 // shadow), accumulate; or the per-sample direct/normals kernel.
 void run_chunk(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
-               const LaunchHooks* hooks);
+               const SideStream* side, const LaunchHooks* hooks);
 void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
                const LaunchHooks* hooks);
 
@@ -45,7 +55,7 @@ struct BatchGrids {
 // One batched pass (FP32 path integrator): generate, max_depth x (extend,
 // shade, plan, shadow), accumulate over the jobs of `jobs` (device table).
 void run_batch(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPlans& L, int depth, uint32_t stack,
-               const BatchGrids& g, hipStream_t s, const LaunchHooks* hooks);
+               const BatchGrids& g, hipStream_t s, const SideStream* side, const LaunchHooks* hooks);
 // Resident workgroups per CU of the batch kernels (for the persistent grids).
 void batch_occupancy(uint32_t stack, uint32_t* trace, uint32_t* shade, uint32_t* other);
 
