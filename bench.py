@@ -320,6 +320,7 @@ def main():
     kernel_ms = {k: round(v["ms"] / args.steps, 3) for k, v in prof.items() if isinstance(v, dict)}
     # The same kernel with the any-hit kernels on the main stream (not timed
     # for `value`): the traversal kernel's own rate.
+    saved = os.environ.get("RTG_OVERLAP")
     os.environ["RTG_OVERLAP"] = "0"
     rtg.set_profiling(ctx, True)
     for _ in range(2):
@@ -327,7 +328,10 @@ def main():
     ctx.synchronize()
     roofline_serial = extend_roofline(rtg.get_profile(ctx).as_dict(), traffic)
     rtg.set_profiling(ctx, False)
-    del os.environ["RTG_OVERLAP"]
+    if saved is None:
+        del os.environ["RTG_OVERLAP"]
+    else:
+        os.environ["RTG_OVERLAP"] = saved
     dev.release()
     acc.release()
     ctx.close()
