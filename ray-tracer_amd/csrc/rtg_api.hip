@@ -39,6 +39,7 @@ struct rtg_context {
   SideStream side{};  // default priority, like the main stream
   std::string err;
   unsigned long long* stats = nullptr;
+  unsigned long long* stats_host = nullptr;  // pinned copy of `stats`, written at the end of a call
   int n_cu = 0;
   // Path state, grown on demand: FP32 chunks, FP64 chunks, FP32 batches.
   void* path_mem[3] = {nullptr, nullptr, nullptr};
@@ -219,11 +220,13 @@ void profile_mark(void* user, int kind, bool begin, hipStream_t stream) {
   hipEventRecord(ctx->events[ctx->events_used++], stream);
 }
 
-// After the stream has drained: add every begin/end pair to the totals.
-void profile_resolve(rtg_context* ctx, const unsigned long long* h) {
-  if (!ctx->profiling) return;
+// Adds every recorded begin/end pair to the totals. Called when the profile
+// is read and when many events are pending, not after every call, so that
+// the host does not read event times while the next pass could be running.
+void profile_flush_events(rtg_context* ctx) {
   for (size_t i = 0; i + 1 < ctx->events_used; i += 2) {
     float ms = 0;
+    hipEventSynchronize(ctx->events[i + 1]);
     if (hipEventElapsedTime(&ms, ctx->events[i], ctx->events[i + 1]) == hipSuccess) {
       const int k = ctx->event_kind[i];
       ctx->profile.ms[k] += ms;
@@ -231,6 +234,14 @@ void profile_resolve(rtg_context* ctx, const unsigned long long* h) {
     }
   }
   ctx->events_used = 0;
+}
+
+// After the stream has drained: a call's counters (its launch times stay
+// pending until profile_flush_events).
+void profile_resolve(rtg_context* ctx, const unsigned long long* h) {
+  if (!ctx->profiling) return;
+  constexpr size_t kPendingEvents = 1u << 13;
+  if (ctx->events_used >= kPendingEvents) profile_flush_events(ctx);
   ctx->profile.closest_rays += h[kStRays];
   ctx->profile.closest_visits += h[kStVisits];
   ctx->profile.closest_tests += h[kStTests];
@@ -269,6 +280,8 @@ rtg_status rtg_set_profiling(rtg_context* ctx, int32_t on) {
 
 rtg_status rtg_get_profile(rtg_context* ctx, rtg_profile* out) {
   if (!ctx || !out) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_get_profile: NULL argument");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  profile_flush_events(ctx);
   *out = ctx->profile;
   return RTG_OK;
 }
@@ -290,6 +303,7 @@ rtg_status rtg_open(int device, rtg_context** out) {
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.shaded, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.shadowed, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&c->stats, sizeof(unsigned long long) * kStatSlots * kStatStride);
+  if (e == hipSuccess) e = hipHostMalloc(&c->stats_host, sizeof(unsigned long long) * kStatSlots * kStatStride);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) {
     const std::string m = std::string("rtg_open: ") + hipGetErrorString(e);
@@ -308,6 +322,7 @@ rtg_status rtg_close(rtg_context* ctx) {
   for (void* p : ctx->path_mem)
     if (p) hipFree(p);
   if (ctx->stats) hipFree(ctx->stats);
+  if (ctx->stats_host) hipHostFree(ctx->stats_host);
   if (ctx->batch_jobs) hipFree(ctx->batch_jobs);
   if (ctx->batch_plans) hipFree(ctx->batch_plans);
   if (ctx->batch_host) hipHostFree(ctx->batch_host);
@@ -727,10 +742,17 @@ void add_counters(rtg_stats* out, const unsigned long long* h) {
   out->prim_tests += h[kStTests] + h[kStSTests];
 }
 
+// Enqueues the copy of the statistics slots into pinned memory (before the
+// call's final event, so one wait covers both).
+rtg_status stage_stats(rtg_context* ctx) {
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->stats_host, ctx->stats, sizeof(unsigned long long) * kStatSlots * kStatStride,
+                              hipMemcpyDeviceToHost, ctx->stream));
+  return RTG_OK;
+}
+
+// After the final event: sums the staged slots.
 void read_stats(rtg_context* ctx, rtg_stats* out, float ms) {
-  static thread_local std::vector<unsigned long long> slots;
-  slots.assign((size_t)kStatSlots * kStatStride, 0);
-  hipMemcpy(slots.data(), ctx->stats, slots.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+  const unsigned long long* slots = ctx->stats_host;
   unsigned long long h[kStCount] = {0};
   for (int sl = 0; sl < kStatSlots; ++sl)
     for (int k = 0; k < kStCount; ++k) h[k] += slots[(size_t)sl * kStatStride + k];
@@ -824,6 +846,7 @@ rtg_status render_range(rtg_context* ctx, const rtg_scene* scene, const rtg_rend
     }
     sa += k;
   }
+  if (rtg_status st = stage_stats(ctx)) return st;
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
   float ms = 0;
@@ -1124,6 +1147,7 @@ rtg_status rtg_trace(rtg_context* ctx, const rtg_scene* scene, const rtg_buffer*
                   static_cast<rtg_hit*>(hits->ptr), ctx->stats, scene->stack, ctx->stream, hooks_of(ctx));
     HIP_TRY(ctx, hipGetLastError());
   }
+  if (rtg_status st = stage_stats(ctx)) return st;
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
   float ms = 0;
