@@ -50,7 +50,7 @@ struct rtg_context {
   uint32_t cap[3] = {0, 0, 0};
   // Batches: device job table and plans, pinned host staging of both.
   BatchJob* batch_jobs = nullptr;
-  uint32_t* batch_plans = nullptr;  // job count, then 4 x (kMaxBatchJobs + 1): gen, acc, act, shd
+  uint32_t* batch_plans = nullptr;  // job count, then 5 x (kMaxBatchJobs + 1): gen, acc, act, shd x 2
   void* batch_host = nullptr;       // job table + job count + gen/acc plans, pinned
   uint32_t batch_occ[3][3] = {};    // per stack class: trace, shade, other workgroups per CU
   // Captured batch passes, one per (max depth, stack class): the launches of
@@ -301,7 +301,8 @@ rtg_status rtg_open(int device, rtg_context** out) {
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side.stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.shaded, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.shadowed, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.shadowed[0], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.shadowed[1], hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&c->stats, sizeof(unsigned long long) * kStatSlots * kStatStride);
   if (e == hipSuccess) e = hipHostMalloc(&c->stats_host, sizeof(unsigned long long) * kStatSlots * kStatStride);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
@@ -334,7 +335,8 @@ rtg_status rtg_close(rtg_context* ctx) {
     hipStreamDestroy(ctx->side.stream);
   }
   if (ctx->side.shaded) hipEventDestroy(ctx->side.shaded);
-  if (ctx->side.shadowed) hipEventDestroy(ctx->side.shadowed);
+  for (hipEvent_t e : ctx->side.shadowed)
+    if (e) hipEventDestroy(e);
   for (hipEvent_t e : ctx->events) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -681,12 +683,15 @@ rtg_status ensure_paths(rtg_context* ctx, int which, uint32_t cap, PathBuffers<T
   auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
   const uint64_t tbytes = al(c * sizeof(T));
   // 6 ray + 3 beta + 3 L + 3 hit + 10 shadow arrays of T; 2 RNG u64; 5 u32
-  // (2 queues, hprim, hplane, sq_pid); spec; counts.
+  // (2 queues, hprim, hplane, sq_pid); spec; counts. FP32: a second shadow
+  // queue (10 T + sq_pid) and 3 shadow-radiance arrays.
+  constexpr bool f32 = sizeof(T) == 4;
+  const uint64_t sq = f32 ? 2 * c : c;  // shadow-queue entries
   const bool batch = which == kBatchBuffers;
   const uint64_t n_counts = batch ? (uint64_t)kMaxBatchJobs * kBatchCounts : 2ull * kMaxBounce * kSubQueues;
   const uint64_t n_stats = batch ? (uint64_t)kMaxBatchJobs * kBatchStatSlots * kStatStride : 0;
-  const uint64_t bytes = tbytes * 25 + al(c * 8) * 2 + al(c * 4) * 5 + al(c) + al(sizeof(uint32_t) * n_counts) +
-                         al(8 * n_stats);
+  const uint64_t bytes = tbytes * (15 + (f32 ? 3 : 0)) + al(sq * sizeof(T)) * 10 + al(c * 8) * 2 + al(c * 4) * 4 +
+                         al(sq * 4) + al(c) + al(sizeof(uint32_t) * n_counts) + al(8 * n_stats);
   void* mem = nullptr;
   HIP_TRY(ctx, hipMalloc(&mem, bytes));
   char* p = static_cast<char*>(mem);
@@ -696,15 +701,24 @@ rtg_status ensure_paths(rtg_context* ctx, int which, uint32_t cap, PathBuffers<T
     return r;
   };
   T** tarr[] = {&B.ox, &B.oy, &B.oz, &B.dx, &B.dy, &B.dz, &B.bx, &B.by, &B.bz, &B.lx, &B.ly, &B.lz, &B.ht,
-                &B.hb1, &B.hb2, &B.sox, &B.soy, &B.soz, &B.sdx, &B.sdy, &B.sdz, &B.stm, &B.scx, &B.scy, &B.scz};
+                &B.hb1, &B.hb2};
   for (T** a : tarr) *a = reinterpret_cast<T*>(take(c * sizeof(T)));
+  T** sarr[] = {&B.sox, &B.soy, &B.soz, &B.sdx, &B.sdy, &B.sdz, &B.stm, &B.scx, &B.scy, &B.scz};
+  for (T** a : sarr) *a = reinterpret_cast<T*>(take(sq * sizeof(T)));
+  B.slx = B.sly = B.slz = nullptr;
+  if (f32) {
+    B.slx = reinterpret_cast<T*>(take(c * sizeof(T)));
+    B.sly = reinterpret_cast<T*>(take(c * sizeof(T)));
+    B.slz = reinterpret_cast<T*>(take(c * sizeof(T)));
+  }
+  B.sq_stride = f32 ? (uint32_t)c : 0u;
   B.rs = reinterpret_cast<uint64_t*>(take(c * 8));
   B.ri = reinterpret_cast<uint64_t*>(take(c * 8));
   B.q[0] = reinterpret_cast<uint32_t*>(take(c * 4));
   B.q[1] = reinterpret_cast<uint32_t*>(take(c * 4));
   B.hprim = reinterpret_cast<uint32_t*>(take(c * 4));
   B.hplane = reinterpret_cast<int32_t*>(take(c * 4));
-  B.sq_pid = reinterpret_cast<uint32_t*>(take(c * 4));
+  B.sq_pid = reinterpret_cast<uint32_t*>(take(sq * 4));
   B.spec = reinterpret_cast<uint8_t*>(take(c));
   B.counts = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * n_counts));
   B.stats = batch ? reinterpret_cast<unsigned long long*>(take(8 * n_stats)) : ctx->stats;
@@ -885,7 +899,7 @@ bool batch_graphs_enabled() {
 rtg_status ensure_batch_tables(rtg_context* ctx) {
   if (ctx->batch_jobs) return RTG_OK;
   HIP_TRY(ctx, hipMalloc(&ctx->batch_jobs, sizeof(BatchJob) * kMaxBatchJobs));
-  HIP_TRY(ctx, hipMalloc(&ctx->batch_plans, sizeof(uint32_t) * (1 + 4 * (kMaxBatchJobs + 1))));
+  HIP_TRY(ctx, hipMalloc(&ctx->batch_plans, sizeof(uint32_t) * (1 + 5 * (kMaxBatchJobs + 1))));
   HIP_TRY(ctx, hipHostMalloc(&ctx->batch_host,
                              sizeof(BatchJob) * kMaxBatchJobs + sizeof(uint32_t) * (1 + 2 * (kMaxBatchJobs + 1))));
   return RTG_OK;

@@ -18,30 +18,32 @@ void chunk_impl(const DevScene& S, const PathBuffers<float>& B, const ChunkParam
   } else {
     RTG_LAUNCH(h, RTG_K_GENERATE, k_generate<float>, dim3(grid_of(generate_threads(P))), dim3(kBlock), 0, s, S, B, P);
     const bool nee = P.light_sampling && S.n_lights > 0;
-    // With a side stream, the any-hit kernel of bounce b runs beside the
-    // closest-hit kernel of bounce b + 1: it reads the shadow queue of
-    // bounce b and adds to the radiance, while extend reads the next active
-    // queue and writes hit records. The shade kernel of bounce b + 1 (which
-    // adds to the radiance and rewrites the shadow queue) and the
-    // accumulation wait for it, so every path's additions keep their order.
-    bool pending = false;
+    // With a side stream, the any-hit kernels run there in bounce order,
+    // beside the closest-hit and shade kernels of the following bounces: the
+    // any-hit kernel of bounce b reads buffer b & 1 of the shadow queue and
+    // adds to the shadow radiance, which nothing else touches until the
+    // accumulation. The shade kernel of bounce b rewrites buffer b & 1, so it
+    // waits for the any-hit kernel of bounce b - 2; the accumulation waits
+    // for the last. Every path's additions keep their order.
+    bool pending[2] = {false, false};
     for (int b = 0; b < P.max_depth; ++b) {
       RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
-      if (pending) hipStreamWaitEvent(s, side->shadowed, 0);
-      pending = false;
+      if (pending[b & 1]) hipStreamWaitEvent(s, side->shadowed[b & 1], 0);
+      pending[b & 1] = false;
       RTG_LAUNCH(h, RTG_K_SHADE, k_shade<float>, dim3(g), dim3(kBlock), 0, s, S, B, P, b);
       if (!nee) continue;
       if (side) {
         hipEventRecord(side->shaded, s);
         hipStreamWaitEvent(side->stream, side->shaded, 0);
         RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<float, STACK>), dim3(g), dim3(kBlock), 0, side->stream, S, B, P, b);
-        hipEventRecord(side->shadowed, side->stream);
-        pending = true;
+        hipEventRecord(side->shadowed[b & 1], side->stream);
+        pending[b & 1] = true;
       } else {
         RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
       }
     }
-    if (pending) hipStreamWaitEvent(s, side->shadowed, 0);
+    for (int p = 0; p < 2; ++p)
+      if (pending[p]) hipStreamWaitEvent(s, side->shadowed[p], 0);
   }
   RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_accumulate<float>, dim3(accumulate_blocks<float>(P)), dim3(kBlock), 0, s, B, P);
 }
@@ -51,14 +53,15 @@ void batch_impl(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPl
                 hipStream_t s, const SideStream* side, const LaunchHooks* h) {
   RTG_LAUNCH(h, RTG_K_GENERATE, k_batch_generate<float>, dim3(g.generate), dim3(kBlock), 0, s, jobs, B, L);
   RTG_LAUNCH(h, RTG_K_PLAN, k_batch_plan<float>, dim3(1), dim3(kMaxBatchJobs), 0, s, jobs, L, -1);
-  // As in a single render, the any-hit kernel of bounce b may run beside the
-  // closest-hit kernel of bounce b + 1; the shade and planning kernels of
-  // bounce b + 1 (the planner rewrites the shadow plan) wait for it.
-  bool pending = false;
+  // As in a single render, the any-hit kernels run on the side stream; the
+  // shade and planning kernels of bounce b (which rewrite buffer b & 1 of
+  // the shadow queue and of the shadow plan) wait for the any-hit kernel of
+  // bounce b - 2.
+  bool pending[2] = {false, false};
   for (int b = 0; b < depth; ++b) {
     RTG_LAUNCH(h, RTG_K_EXTEND, (k_batch_extend<float, STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
-    if (pending) hipStreamWaitEvent(s, side->shadowed, 0);
-    pending = false;
+    if (pending[b & 1]) hipStreamWaitEvent(s, side->shadowed[b & 1], 0);
+    pending[b & 1] = false;
     RTG_LAUNCH(h, RTG_K_SHADE, k_batch_shade<float>, dim3(g.shade), dim3(kBlock), 0, s, jobs, B, L, b);
     RTG_LAUNCH(h, RTG_K_PLAN, k_batch_plan<float>, dim3(1), dim3(kMaxBatchJobs), 0, s, jobs, L, b);
     if (side) {
@@ -66,13 +69,14 @@ void batch_impl(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPl
       hipStreamWaitEvent(side->stream, side->shaded, 0);
       RTG_LAUNCH(h, RTG_K_SHADOW, (k_batch_shadow<float, STACK>), dim3(g.trace), dim3(kBlock), 0, side->stream, jobs,
                  B, L, b);
-      hipEventRecord(side->shadowed, side->stream);
-      pending = true;
+      hipEventRecord(side->shadowed[b & 1], side->stream);
+      pending[b & 1] = true;
     } else {
       RTG_LAUNCH(h, RTG_K_SHADOW, (k_batch_shadow<float, STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
     }
   }
-  if (pending) hipStreamWaitEvent(s, side->shadowed, 0);
+  for (int p = 0; p < 2; ++p)
+    if (pending[p]) hipStreamWaitEvent(s, side->shadowed[p], 0);
   RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_batch_accumulate<float>, dim3(g.accumulate), dim3(kBlock), 0, s, jobs, B, L);
 }
 

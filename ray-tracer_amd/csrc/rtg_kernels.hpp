@@ -39,6 +39,13 @@ struct PathBuffers {
   uint32_t* counts;                // sub-queue sizes: [(b)*kSubQueues + s] active, [(kMaxBounce+b)*kSubQueues + s] shadow
   uint32_t* sq_pid;                // shadow queue (same sub-queue layout)
   T *sox, *soy, *soz, *sdx, *sdy, *sdz, *stm, *scx, *scy, *scz;
+  // FP32: the shadow queue has two buffers (bounce parity; the second at
+  // sq_stride) and shadow-ray radiance is summed apart from the path's own
+  // (slx..: added to L when the path is accumulated), so the any-hit kernel
+  // of bounce b can run beside the shade kernel of bounce b + 1. FP64 keeps
+  // one buffer (sq_stride 0) and adds to L in the reference's order.
+  T *slx, *sly, *slz;
+  uint32_t sq_stride;
   unsigned long long* stats;       // kStatSlots slots of kStatStride counters
   uint32_t capacity;               // paths per chunk
   uint32_t sub_cap;                // entries per sub-queue
@@ -317,6 +324,9 @@ __device__ __forceinline__ void generate_body(const DevScene& S, const PathBuffe
   B.dx[pid] = d.x; B.dy[pid] = d.y; B.dz[pid] = d.z;
   B.bx[pid] = 1; B.by[pid] = 1; B.bz[pid] = 1;
   B.lx[pid] = 0; B.ly[pid] = 0; B.lz[pid] = 0;
+  if constexpr (sizeof(T) == 4) {
+    B.slx[pid] = 0; B.sly[pid] = 0; B.slz[pid] = 0;
+  }
   B.spec[pid] = 1;
 }
 template <class T>
@@ -433,7 +443,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
                                       lb);
   if (alive) B.q[(b + 1) & 1][P.q_base + a.a] = pid;
   if (shadow) {
-    const int64_t si = P.q_base + a.b;
+    const int64_t si = P.q_base + a.b + (int64_t)(b & 1) * B.sq_stride;
     B.sq_pid[si] = pid;
     B.sox[si] = req.o.x; B.soy[si] = req.o.y; B.soz[si] = req.o.z;
     B.sdx[si] = req.d.x; B.sdy[si] = req.d.y; B.sdz[si] = req.d.z;
@@ -458,15 +468,21 @@ __device__ __forceinline__ void shadow_body(const DevScene& S, const PathBuffers
   const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
   if (i < qv.total) {
-    const uint32_t k = P.q_base + queue_slot(pre, P.sub_cap, i);
+    const uint32_t k = P.q_base + queue_slot(pre, P.sub_cap, i) + (uint32_t)(b & 1) * B.sq_stride;
     const uint32_t pid = B.sq_pid[k];
     const V3<T> o{B.sox[k], B.soy[k], B.soz[k]}, d{B.sdx[k], B.sdy[k], B.sdz[k]};
     Counters c{0, 0};
     const bool occ = ModeOps<T>::occluded(S, P.use_bvh != 0, o, d, B.stm[k], lds_stack(sref), c);
     if (!occ) {
-      B.lx[pid] = B.lx[pid] + B.scx[k];
-      B.ly[pid] = B.ly[pid] + B.scy[k];
-      B.lz[pid] = B.lz[pid] + B.scz[k];
+      if constexpr (sizeof(T) == 4) {
+        B.slx[pid] = B.slx[pid] + B.scx[k];
+        B.sly[pid] = B.sly[pid] + B.scy[k];
+        B.slz[pid] = B.slz[pid] + B.scz[k];
+      } else {
+        B.lx[pid] = B.lx[pid] + B.scx[k];
+        B.ly[pid] = B.ly[pid] + B.scy[k];
+        B.lz[pid] = B.lz[pid] + B.scz[k];
+      }
     }
     st[kStSVisits] = c.visits;
     st[kStSTests] = c.tests;
@@ -507,11 +523,16 @@ __host__ __device__ inline uint32_t accumulate_blocks(const ChunkParams& P) {
 
 template <class T>
 __device__ __forceinline__ void add_pixel(const ChunkParams& P, uint32_t pixel, const T* x, const T* y, const T* z,
-                                          uint32_t stride, uint32_t* st) {
+                                          const T* xs, const T* ys, const T* zs, uint32_t stride, uint32_t* st) {
   double sx = P.acc_sum[3 * (size_t)pixel], sy = P.acc_sum[3 * (size_t)pixel + 1],
          sz = P.acc_sum[3 * (size_t)pixel + 2];
   for (uint32_t s = 0; s < P.k; ++s) {
     T a = x[s * stride], b = y[s * stride], c = z[s * stride];
+    if (xs) {  // FP32 path integrator: plus the shadow-ray radiance
+      a = a + xs[s * stride];
+      b = b + ys[s * stride];
+      c = c + zs[s * stride];
+    }
     if (!(r_isfinite(a) && r_isfinite(b) && r_isfinite(c)) || a < 0 || b < 0 || c < 0) {
       ++st[kStInvalid];
       a = b = c = 0;
@@ -530,11 +551,13 @@ __device__ __forceinline__ void add_pixel(const ChunkParams& P, uint32_t pixel, 
 template <class T>
 __device__ __forceinline__ void accumulate_body(const PathBuffers<T>& B, const ChunkParams& P, uint32_t lb, T* tile) {
   uint32_t st[kStCount] = {0};
+  const bool ls = sizeof(T) == 4 && P.integrator == RTG_INTEGRATOR_PATH;  // shadow radiance kept apart
   if (P.sample_major || P.k > kAccTile<T>) {
     const uint32_t j = lb * kBlock + threadIdx.x;
     if (j < P.n_pix) {
       const uint32_t p0 = pid_of(P, j, 0), stride = P.sample_major ? P.n_pix : 1u;
-      add_pixel<T>(P, P.p0 + j, B.lx + p0, B.ly + p0, B.lz + p0, stride, st);
+      add_pixel<T>(P, P.p0 + j, B.lx + p0, B.ly + p0, B.lz + p0, ls ? B.slx + p0 : nullptr, ls ? B.sly + p0 : nullptr,
+                   ls ? B.slz + p0 : nullptr, stride, st);
     }
   } else {
     const uint32_t ppb = kAccTile<T> / P.k, j0 = lb * ppb;
@@ -545,14 +568,14 @@ __device__ __forceinline__ void accumulate_body(const PathBuffers<T>& B, const C
     T* tz = tile + 4 * kAccTile<T>;
     for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
       const uint32_t at = e + e / P.k;  // pixel e / k at (k + 1) * (e / k)
-      tx[at] = B.lx[base + e];
-      ty[at] = B.ly[base + e];
-      tz[at] = B.lz[base + e];
+      tx[at] = ls ? B.lx[base + e] + B.slx[base + e] : B.lx[base + e];
+      ty[at] = ls ? B.ly[base + e] + B.sly[base + e] : B.ly[base + e];
+      tz[at] = ls ? B.lz[base + e] + B.slz[base + e] : B.lz[base + e];
     }
     __syncthreads();
     for (uint32_t jj = threadIdx.x; jj < np; jj += kBlock) {
       const uint32_t at = jj * (P.k + 1);
-      add_pixel<T>(P, P.p0 + j0 + jj, tx + at, ty + at, tz + at, 1u, st);
+      add_pixel<T>(P, P.p0 + j0 + jj, tx + at, ty + at, tz + at, nullptr, nullptr, nullptr, 1u, st);
     }
     __syncthreads();  // the tile is reused by the block's next job
   }
@@ -809,7 +832,7 @@ struct BatchPlans {
   const uint32_t* gen;     // written by the host
   const uint32_t* acc;     // written by the host
   uint32_t* act;           // active queue of the next extend/shade (k_batch_plan)
-  uint32_t* shd;           // shadow queue of the next shadow step (k_batch_plan)
+  uint32_t* shd;           // shadow queues of the shadow steps (k_batch_plan), one plan per bounce parity
 };
 // The job count is read from device memory, so a captured sequence of batch
 // launches (a graph) replays for any number of jobs.
@@ -867,13 +890,14 @@ __global__ __launch_bounds__(kMaxBatchJobs) void k_batch_plan(const BatchJob* __
     ss[j] += c;
     __syncthreads();
   }
+  uint32_t* shd = L.shd + (b & 1) * (kMaxBatchJobs + 1);
   if (j < n_jobs) {
     L.act[j + 1] = sa[j];
-    L.shd[j + 1] = ss[j];
+    shd[j + 1] = ss[j];
   }
   if (j == 0) {
     L.act[0] = 0;
-    L.shd[0] = 0;
+    shd[0] = 0;
   }
 }
 
@@ -905,7 +929,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_shadow(const BatchJob* __restr
                                                          BatchPlans L, int b) {
   __shared__ uint32_t sref[STACK * kBlock];
   __shared__ uint32_t pre[kSubQueues + 1];
-  batch_loop(L.shd, jobs_of(L), [&](uint32_t j, uint32_t lb) {
+  batch_loop(L.shd + (b & 1) * (kMaxBatchJobs + 1), jobs_of(L), [&](uint32_t j, uint32_t lb) {
     shadow_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, pre);
   });
 }

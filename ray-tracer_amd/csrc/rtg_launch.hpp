@@ -32,13 +32,15 @@ struct LaunchHooks {
   } while (0)
 
 // A second stream for work of a pass that may overlap the main stream's:
-// the any-hit kernel of bounce b runs beside the closest-hit kernel of
-// bounce b + 1 (they touch disjoint path state; the shade kernel of bounce
-// b + 1 waits for both). Null: everything on the main stream.
+// the any-hit kernels run there, in bounce order, beside the closest-hit and
+// shade kernels of the following bounces (FP32: they read their own shadow
+// queue buffer and add to the shadow radiance only). The shade kernel of
+// bounce b waits for the any-hit kernel of bounce b - 2, which read the same
+// buffer. Null: everything on the main stream.
 struct SideStream {
   hipStream_t stream;
-  hipEvent_t shaded;    // main stream: shade of the current bounce done
-  hipEvent_t shadowed;  // side stream: any-hit of the current bounce done
+  hipEvent_t shaded;       // main stream: shade of the current bounce done
+  hipEvent_t shadowed[2];  // side stream: any-hit of the last bounce of each parity done
 };
 
 // Runs one chunk of a render pass: generate, max_depth x (extend, shade,
