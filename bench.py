@@ -315,23 +315,26 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = extend_roofline(prof, traffic)
-    roofline["concurrent"] = ("the any-hit kernel of the previous bounce runs beside each launch on a second "
-                              "stream, so a launch's duration includes time shared with it")
+    roofline["concurrent"] = ("a pass runs as two chunks on two streams, each with its any-hit kernels on a "
+                              "third or fourth stream, so a launch shares the GPU with up to three other kernels "
+                              "and its duration includes that time; roofline_serial is the kernel alone")
     kernel_ms = {k: round(v["ms"] / args.steps, 3) for k, v in prof.items() if isinstance(v, dict)}
-    # The same kernel with the any-hit kernels on the main stream (not timed
+    # The same kernel with every launch of a pass on one stream (not timed
     # for `value`): the traversal kernel's own rate.
-    saved = os.environ.get("RTG_OVERLAP")
+    saved = {k: os.environ.get(k) for k in ("RTG_OVERLAP", "RTG_LANES")}
     os.environ["RTG_OVERLAP"] = "0"
+    os.environ["RTG_LANES"] = "1"
     rtg.set_profiling(ctx, True)
     for _ in range(2):
         render_once(rtg, ctx, dev, acc, cfg, spp)
     ctx.synchronize()
     roofline_serial = extend_roofline(rtg.get_profile(ctx).as_dict(), traffic)
     rtg.set_profiling(ctx, False)
-    if saved is None:
-        del os.environ["RTG_OVERLAP"]
-    else:
-        os.environ["RTG_OVERLAP"] = saved
+    for k, v in saved.items():
+        if v is None:
+            del os.environ[k]
+        else:
+            os.environ[k] = v
     dev.release()
     acc.release()
     ctx.close()

@@ -21,6 +21,7 @@ thread_local std::string g_error;
 
 // Path-state slots of a context.
 constexpr int kBatchBuffers = 2;
+constexpr int kLaneBuffers = 3;  // FP32 chunks of the second lane
 // Batches: every job's queue storage is rounded up to kSubQueues sub-queues
 // of whole workgroups, at most kSubQueues * kBlock entries more than its
 // paths.
@@ -42,12 +43,19 @@ struct rtg_context {
   unsigned long long* stats_host = nullptr;  // pinned copy of `stats`, written at the end of a call
   int n_cu = 0;
   // Path state, grown on demand: FP32 chunks, FP64 chunks, FP32 batches.
-  void* path_mem[3] = {nullptr, nullptr, nullptr};
-  uint64_t path_bytes[3] = {0, 0, 0};
+  void* path_mem[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint64_t path_bytes[4] = {0, 0, 0, 0};
   PathBuffers<float> pb32{};
+  // A second lane for FP32 path-integrator passes: chunks alternate between
+  // (stream, side, pb32) and (stream2, side2, pb32b), so two chunks' kernels
+  // interleave and fill each other's tails.
+  hipStream_t stream2 = nullptr;
+  SideStream side2{};
+  PathBuffers<float> pb32b{};
+  hipEvent_t fork = nullptr, join = nullptr;
   PathBuffers<double> pb64{};
   PathBuffers<float> pbb{};
-  uint32_t cap[3] = {0, 0, 0};
+  uint32_t cap[4] = {0, 0, 0, 0};
   // Batches: device job table and plans, pinned host staging of both.
   BatchJob* batch_jobs = nullptr;
   uint32_t* batch_plans = nullptr;  // job count, then 5 x (kMaxBatchJobs + 1): gen, acc, act, shd x 2
@@ -252,9 +260,9 @@ void profile_resolve(rtg_context* ctx, const unsigned long long* h) {
 
 // The side stream for overlapping a pass's any-hit kernels, unless
 // RTG_OVERLAP=0 (everything on the main stream).
-const SideStream* side_of(rtg_context* ctx) {
+const SideStream* side_of(rtg_context* ctx, bool lane2 = false) {
   const char* e = std::getenv("RTG_OVERLAP");
-  return (e && e[0] == '0') ? nullptr : &ctx->side;
+  return (e && e[0] == '0') ? nullptr : lane2 ? &ctx->side2 : &ctx->side;
 }
 
 const LaunchHooks* hooks_of(rtg_context* ctx) {
@@ -303,6 +311,13 @@ rtg_status rtg_open(int device, rtg_context** out) {
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.shaded, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.shadowed[0], hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.shadowed[1], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side2.stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side2.shaded, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side2.shadowed[0], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side2.shadowed[1], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&c->stats, sizeof(unsigned long long) * kStatSlots * kStatStride);
   if (e == hipSuccess) e = hipHostMalloc(&c->stats_host, sizeof(unsigned long long) * kStatSlots * kStatStride);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
@@ -336,6 +351,13 @@ rtg_status rtg_close(rtg_context* ctx) {
   }
   if (ctx->side.shaded) hipEventDestroy(ctx->side.shaded);
   for (hipEvent_t e : ctx->side.shadowed)
+    if (e) hipEventDestroy(e);
+  for (hipStream_t st : {ctx->stream2, ctx->side2.stream})
+    if (st) {
+      hipStreamSynchronize(st);
+      hipStreamDestroy(st);
+    }
+  for (hipEvent_t e : {ctx->side2.shaded, ctx->side2.shadowed[0], ctx->side2.shadowed[1], ctx->fork, ctx->join})
     if (e) hipEventDestroy(e);
   for (hipEvent_t e : ctx->events) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -673,6 +695,7 @@ rtg_status ensure_paths(rtg_context* ctx, int which, uint32_t cap, PathBuffers<T
   if (which == kBatchBuffers) drop_batch_graphs(ctx);
   if (ctx->path_mem[which]) {
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream2));
     HIP_TRY(ctx, hipFree(ctx->path_mem[which]));
     ctx->path_mem[which] = nullptr;
     ctx->cap[which] = 0;
@@ -817,47 +840,82 @@ uint32_t capacity_for(uint64_t paths) {
   return std::min(cap, lim);
 }
 
+// Chunks of a pass on two lanes (FP32 path integrator) unless RTG_LANES=1.
+bool two_lanes_enabled() {
+  const char* e = std::getenv("RTG_LANES");
+  return !(e && e[0] == '1');
+}
+
 // Samples [s0, s1) of pixels [px0, px1), one chunk of whole pixels at a time.
+// FP32 path-integrator chunks alternate between two lanes (streams, path
+// buffers); the chunks of one sample sub-range cover disjoint pixels, and
+// the lanes join before the next sub-range, so every pixel's additions stay
+// in sample order.
 rtg_status render_range(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* cfg, uint32_t s0,
                         uint32_t s1, uint32_t px0, uint32_t px1, rtg_accum* acc, const volatile int32_t* cancel,
                         rtg_stats* stats, const char* fn) {
   const bool f64 = cfg->precision == RTG_PRECISION_F64;
-  const uint32_t cap = capacity_for((uint64_t)(s1 - s0) * (px1 - px0));
+  const uint64_t paths = (uint64_t)(s1 - s0) * (px1 - px0);
+  const bool two = !f64 && cfg->integrator == RTG_INTEGRATOR_PATH && two_lanes_enabled();
+  const uint32_t cap = capacity_for(two ? (paths + 1) / 2 : paths);
   if (f64) {
     if (rtg_status st = ensure_paths(ctx, 1, cap, ctx->pb64)) return st;
   } else {
     if (rtg_status st = ensure_paths(ctx, 0, cap, ctx->pb32)) return st;
+    if (two)
+      if (rtg_status st = ensure_paths(ctx, kLaneBuffers, cap, ctx->pb32b)) return st;
   }
   HIP_TRY(ctx, hipMemsetAsync(ctx->stats, 0, sizeof(unsigned long long) * kStatSlots * kStatStride, ctx->stream));
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  auto fork = [&]() -> rtg_status {  // the second lane starts after the main stream's work so far
+    HIP_TRY(ctx, hipEventRecord(ctx->fork, ctx->stream));
+    HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream2, ctx->fork, 0));
+    return RTG_OK;
+  };
+  auto join = [&]() -> rtg_status {  // the main stream continues after the second lane's work
+    HIP_TRY(ctx, hipEventRecord(ctx->join, ctx->stream2));
+    HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->join, 0));
+    return RTG_OK;
+  };
   const uint32_t k_total = s1 - s0;
   bool cancelled = false;
   // Sample sub-ranges outermost keep every pixel's additions in sample order.
   for (uint32_t sa = s0; sa < s1 && !cancelled;) {
     const uint32_t k = std::min(k_total - (sa - s0), cap);
     const uint32_t pix_per_chunk = std::max(1u, cap / k);
-    for (uint32_t p0 = px0; p0 < px1; p0 += pix_per_chunk) {
+    if (two)
+      if (rtg_status st = fork()) return st;
+    uint32_t chunk = 0;
+    for (uint32_t p0 = px0; p0 < px1; p0 += pix_per_chunk, ++chunk) {
       if (cancel && *cancel) {
         cancelled = true;
         break;
       }
+      const bool lane2 = two && (chunk & 1);
+      const PathBuffers<float>& B32 = lane2 ? ctx->pb32b : ctx->pb32;
       ChunkParams P = render_params(cfg, acc);
       P.p0 = p0;
       P.k = k;
       P.s0 = sa;
       P.n_pix = std::min(pix_per_chunk, px1 - p0);
       P.n_paths = P.n_pix * k;
-      const PathBuffers<float>& B32 = ctx->pb32;
       P.sub_cap = f64 ? ctx->pb64.sub_cap : B32.sub_cap;
       P.shadow_counts = kMaxBounce * kSubQueues;
       P.stat_slots = kStatSlots;
       P.counts = f64 ? ctx->pb64.counts : B32.counts;
       P.stats = ctx->stats;
       if (f64) run_chunk(scene->dev, ctx->pb64, P, scene->stack, ctx->stream, hooks_of(ctx));
-      else run_chunk(scene->dev, B32, P, scene->stack, ctx->stream, side_of(ctx), hooks_of(ctx));
+      else
+        run_chunk(scene->dev, B32, P, scene->stack, lane2 ? ctx->stream2 : ctx->stream,
+                  side_of(ctx, lane2), hooks_of(ctx));
       HIP_TRY(ctx, hipGetLastError());
-      if (cancel) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // poll between chunks
+      if (cancel) {  // poll between chunks
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (two) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream2));
+      }
     }
+    if (two)
+      if (rtg_status st = join()) return st;
     sa += k;
   }
   if (rtg_status st = stage_stats(ctx)) return st;
