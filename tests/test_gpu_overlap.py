@@ -1,8 +1,10 @@
-"""The any-hit kernel of bounce b runs on a second stream beside the
-closest-hit kernel of bounce b + 1 (single renders and batched passes). The
-shade kernel of bounce b + 1 waits for it, so every path's radiance
-additions keep their order: renders with the overlap and without it
-(RTG_OVERLAP=0) must be bitwise identical, counters included."""
+"""Concurrency inside a pass: the any-hit kernels run on a side stream beside
+the following bounces' kernels (single renders and batched passes), and an
+FP32 pass runs as chunks alternating between two lanes (stream pairs with
+their own path buffers). Every path's additions keep their order and every
+pixel's chunks are added in sample order, so renders must be bitwise
+identical with and without either (RTG_OVERLAP=0, RTG_LANES=1), counters
+included, also when a pass is cut into many small chunks."""
 import numpy as np
 import pytest
 
@@ -30,16 +32,22 @@ def test_single_render_overlap_is_invisible(ctx, monkeypatch, name, w, h):
     o = orc.OracleScene(scenes.load(name), width=w, height=h)
     cfg = o.settings(rtg.PRECISION_F32)
     passes = [(0, 1), (1, 2), (2, 4), (4, 8)]
-    monkeypatch.setenv("RTG_OVERLAP", "1")
-    s1, c1, st1 = render(ctx, o, cfg, passes)
     monkeypatch.setenv("RTG_OVERLAP", "0")
+    monkeypatch.setenv("RTG_LANES", "1")
     s0, c0, st0 = render(ctx, o, cfg, passes)
-    assert np.array_equal(c1, c0)
-    assert np.array_equal(s1.view(np.uint64), s0.view(np.uint64))
-    for a, b in zip(st1, st0):
-        for k in COUNTERS:
-            assert getattr(a, k) == getattr(b, k), k
-    assert sum(a.shadow_rays for a in st1) > 0
+    assert sum(a.shadow_rays for a in st0) > 0
+    for overlap, lanes, chunk in (("1", "1", None), ("0", "2", None), ("1", "2", None), ("1", "2", "4096")):
+        monkeypatch.setenv("RTG_OVERLAP", overlap)
+        monkeypatch.setenv("RTG_LANES", lanes)
+        if chunk:
+            monkeypatch.setenv("RTG_CHUNK_PATHS", chunk)
+        s1, c1, st1 = render(ctx, o, cfg, passes)
+        case = (overlap, lanes, chunk)
+        assert np.array_equal(c1, c0), case
+        assert np.array_equal(s1.view(np.uint64), s0.view(np.uint64)), case
+        for a, b in zip(st1, st0):
+            for k in COUNTERS:
+                assert getattr(a, k) == getattr(b, k), (case, k)
 
 
 def test_batched_overlap_is_invisible(ctx, monkeypatch):
