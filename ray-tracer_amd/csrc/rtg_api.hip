@@ -840,7 +840,10 @@ uint32_t capacity_for(uint64_t paths) {
   return std::min(cap, lim);
 }
 
-// Chunks of a pass on two lanes (FP32 path integrator) unless RTG_LANES=1.
+// Chunks of a pass on two lanes (FP32 path integrator, passes of at least
+// kTwoLanePaths paths, or of one full chunk if chunks are smaller) unless
+// RTG_LANES=1.
+constexpr uint64_t kTwoLanePaths = 1ull << 22;
 bool two_lanes_enabled() {
   const char* e = std::getenv("RTG_LANES");
   return !(e && e[0] == '1');
@@ -856,7 +859,9 @@ rtg_status render_range(rtg_context* ctx, const rtg_scene* scene, const rtg_rend
                         rtg_stats* stats, const char* fn) {
   const bool f64 = cfg->precision == RTG_PRECISION_F64;
   const uint64_t paths = (uint64_t)(s1 - s0) * (px1 - px0);
-  const bool two = !f64 && cfg->integrator == RTG_INTEGRATOR_PATH && two_lanes_enabled();
+  // Small passes stay on one lane: splitting them only adds launches.
+  const bool two = !f64 && cfg->integrator == RTG_INTEGRATOR_PATH &&
+                   paths >= std::min<uint64_t>(kTwoLanePaths, chunk_paths_default()) && two_lanes_enabled();
   const uint32_t cap = capacity_for(two ? (paths + 1) / 2 : paths);
   if (f64) {
     if (rtg_status st = ensure_paths(ctx, 1, cap, ctx->pb64)) return st;

@@ -36,7 +36,7 @@ def test_single_render_overlap_is_invisible(ctx, monkeypatch, name, w, h):
     monkeypatch.setenv("RTG_LANES", "1")
     s0, c0, st0 = render(ctx, o, cfg, passes)
     assert sum(a.shadow_rays for a in st0) > 0
-    for overlap, lanes, chunk in (("1", "1", None), ("0", "2", None), ("1", "2", None), ("1", "2", "4096")):
+    for overlap, lanes, chunk in (("1", "1", None), ("0", "2", "4096"), ("1", "2", "4096"), ("1", "2", "1024")):
         monkeypatch.setenv("RTG_OVERLAP", overlap)
         monkeypatch.setenv("RTG_LANES", lanes)
         if chunk:
