@@ -21,7 +21,8 @@ thread_local std::string g_error;
 
 // Path-state slots of a context.
 constexpr int kBatchBuffers = 2;
-constexpr int kLaneBuffers = 3;  // FP32 chunks of the second lane
+constexpr int kLaneBuffers = 3;    // FP32 chunks of the second lane
+constexpr int kLane64Buffers = 4;  // FP64 chunks of the second lane
 // Batches: every job's queue storage is rounded up to kSubQueues sub-queues
 // of whole workgroups, at most kSubQueues * kBlock entries more than its
 // paths.
@@ -43,8 +44,8 @@ struct rtg_context {
   unsigned long long* stats_host = nullptr;  // pinned copy of `stats`, written at the end of a call
   int n_cu = 0;
   // Path state, grown on demand: FP32 chunks, FP64 chunks, FP32 batches.
-  void* path_mem[4] = {nullptr, nullptr, nullptr, nullptr};
-  uint64_t path_bytes[4] = {0, 0, 0, 0};
+  void* path_mem[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  uint64_t path_bytes[5] = {0, 0, 0, 0, 0};
   PathBuffers<float> pb32{};
   // A second lane for FP32 path-integrator passes: chunks alternate between
   // (stream, side, pb32) and (stream2, side2, pb32b), so two chunks' kernels
@@ -52,10 +53,11 @@ struct rtg_context {
   hipStream_t stream2 = nullptr;
   SideStream side2{};
   PathBuffers<float> pb32b{};
+  PathBuffers<double> pb64b{};
   hipEvent_t fork = nullptr, join = nullptr;
   PathBuffers<double> pb64{};
   PathBuffers<float> pbb{};
-  uint32_t cap[4] = {0, 0, 0, 0};
+  uint32_t cap[5] = {0, 0, 0, 0, 0};
   // Batches: device job table and plans, pinned host staging of both.
   BatchJob* batch_jobs = nullptr;
   uint32_t* batch_plans = nullptr;  // job count, then 5 x (kMaxBatchJobs + 1): gen, acc, act, shd x 2
@@ -840,7 +842,7 @@ uint32_t capacity_for(uint64_t paths) {
   return std::min(cap, lim);
 }
 
-// Chunks of a pass on two lanes (FP32 path integrator, passes of at least
+// Chunks of a pass on two lanes (path integrator, passes of at least
 // kTwoLanePaths paths, or of one full chunk if chunks are smaller) unless
 // RTG_LANES=1.
 constexpr uint64_t kTwoLanePaths = 1ull << 22;
@@ -850,7 +852,7 @@ bool two_lanes_enabled() {
 }
 
 // Samples [s0, s1) of pixels [px0, px1), one chunk of whole pixels at a time.
-// FP32 path-integrator chunks alternate between two lanes (streams, path
+// Path-integrator chunks alternate between two lanes (streams, path
 // buffers); the chunks of one sample sub-range cover disjoint pixels, and
 // the lanes join before the next sub-range, so every pixel's additions stay
 // in sample order.
@@ -860,11 +862,13 @@ rtg_status render_range(rtg_context* ctx, const rtg_scene* scene, const rtg_rend
   const bool f64 = cfg->precision == RTG_PRECISION_F64;
   const uint64_t paths = (uint64_t)(s1 - s0) * (px1 - px0);
   // Small passes stay on one lane: splitting them only adds launches.
-  const bool two = !f64 && cfg->integrator == RTG_INTEGRATOR_PATH &&
+  const bool two = cfg->integrator == RTG_INTEGRATOR_PATH &&
                    paths >= std::min<uint64_t>(kTwoLanePaths, chunk_paths_default()) && two_lanes_enabled();
   const uint32_t cap = capacity_for(two ? (paths + 1) / 2 : paths);
   if (f64) {
     if (rtg_status st = ensure_paths(ctx, 1, cap, ctx->pb64)) return st;
+    if (two)
+      if (rtg_status st = ensure_paths(ctx, kLane64Buffers, cap, ctx->pb64b)) return st;
   } else {
     if (rtg_status st = ensure_paths(ctx, 0, cap, ctx->pb32)) return st;
     if (two)
@@ -898,18 +902,21 @@ rtg_status render_range(rtg_context* ctx, const rtg_scene* scene, const rtg_rend
       }
       const bool lane2 = two && (chunk & 1);
       const PathBuffers<float>& B32 = lane2 ? ctx->pb32b : ctx->pb32;
+      const PathBuffers<double>& B64 = lane2 ? ctx->pb64b : ctx->pb64;
       ChunkParams P = render_params(cfg, acc);
       P.p0 = p0;
       P.k = k;
       P.s0 = sa;
       P.n_pix = std::min(pix_per_chunk, px1 - p0);
       P.n_paths = P.n_pix * k;
-      P.sub_cap = f64 ? ctx->pb64.sub_cap : B32.sub_cap;
+      P.sub_cap = f64 ? B64.sub_cap : B32.sub_cap;
       P.shadow_counts = kMaxBounce * kSubQueues;
       P.stat_slots = kStatSlots;
-      P.counts = f64 ? ctx->pb64.counts : B32.counts;
+      P.counts = f64 ? B64.counts : B32.counts;
       P.stats = ctx->stats;
-      if (f64) run_chunk(scene->dev, ctx->pb64, P, scene->stack, ctx->stream, hooks_of(ctx));
+      if (f64)
+        run_chunk(scene->dev, B64, P, scene->stack, lane2 ? ctx->stream2 : ctx->stream, side_of(ctx, lane2),
+                  hooks_of(ctx));
       else
         run_chunk(scene->dev, B32, P, scene->stack, lane2 ? ctx->stream2 : ctx->stream,
                   side_of(ctx, lane2), hooks_of(ctx));

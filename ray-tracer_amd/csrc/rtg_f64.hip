@@ -47,18 +47,35 @@ __global__ void k_resolve(const double* sum, const uint32_t* count, int n, int o
 namespace {
 template <int STACK>
 void chunk_impl(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, hipStream_t s,
-                const LaunchHooks* h) {
+                const SideStream* side, const LaunchHooks* h) {
   const uint32_t g = grid_of(P.n_paths);
   if (P.integrator != RTG_INTEGRATOR_PATH) {
     RTG_LAUNCH(h, RTG_K_DIRECT, (k_direct<double, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P);
   } else {
     RTG_LAUNCH(h, RTG_K_GENERATE, k_generate<double>, dim3(grid_of(generate_threads(P))), dim3(kBlock), 0, s, S, B, P);
     const bool nee = P.light_sampling && S.n_lights > 0;
+    // With a side stream, the any-hit kernel of bounce b runs beside the
+    // closest-hit kernel of bounce b + 1 only: it adds to the radiance and
+    // reads the one shadow-queue buffer, so the shade kernel of bounce b + 1
+    // waits for it and the additions keep the reference's order.
+    bool pending = false;
     for (int b = 0; b < P.max_depth; ++b) {
       RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend<double, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      if (pending) hipStreamWaitEvent(s, side->shadowed[0], 0);
+      pending = false;
       RTG_LAUNCH(h, RTG_K_SHADE, k_shade<double>, dim3(g), dim3(kBlock), 0, s, S, B, P, b);
-      if (nee) RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<double, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      if (!nee) continue;
+      if (side) {
+        hipEventRecord(side->shaded, s);
+        hipStreamWaitEvent(side->stream, side->shaded, 0);
+        RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<double, STACK>), dim3(g), dim3(kBlock), 0, side->stream, S, B, P, b);
+        hipEventRecord(side->shadowed[0], side->stream);
+        pending = true;
+      } else {
+        RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<double, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      }
     }
+    if (pending) hipStreamWaitEvent(s, side->shadowed[0], 0);
   }
   RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_accumulate<double>, dim3(accumulate_blocks<double>(P)), dim3(kBlock), 0, s, B, P);
 }
@@ -75,11 +92,11 @@ void trace_impl(const DevScene& S, const double* rays, uint64_t n, int mode, int
 // so its stack, kept in LDS, needs the BVH depth in entries: the FP32 stack
 // classes serve it.
 void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
-               const LaunchHooks* h) {
+               const SideStream* side, const LaunchHooks* h) {
   switch (stack_class_of(stack)) {
-    case 0: chunk_impl<19>(S, B, P, s, h); break;
-    case 1: chunk_impl<31>(S, B, P, s, h); break;
-    default: chunk_impl<63>(S, B, P, s, h); break;
+    case 0: chunk_impl<19>(S, B, P, s, side, h); break;
+    case 1: chunk_impl<31>(S, B, P, s, side, h); break;
+    default: chunk_impl<63>(S, B, P, s, side, h); break;
   }
 }
 

@@ -48,6 +48,7 @@ struct SideStream {
 void run_chunk(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
                const SideStream* side, const LaunchHooks* hooks);
 void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
+               const SideStream* side,
                const LaunchHooks* hooks);
 
 // Grid sizes of one batched pass (persistent workgroups; see k_batch_*).
