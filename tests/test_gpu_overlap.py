@@ -29,9 +29,14 @@ def render(ctx, o, cfg, passes):
 @pytest.mark.parametrize("name,w,h,precision", [("torus_c2.json", 160, 120, rtg.PRECISION_F32),
                                                 ("sphere_field_c3.json", 128, 72, rtg.PRECISION_F32),
                                                 ("cornell_c1.json", 96, 96, rtg.PRECISION_F32),
-                                                ("torus_c2.json", 64, 48, rtg.PRECISION_F64)])
+                                                ("torus_c2.json", 64, 48, rtg.PRECISION_F64),
+                                                ("sphere_field_c3.json", 96, 54, "direct")])
 def test_single_render_overlap_is_invisible(ctx, monkeypatch, name, w, h, precision):
-    o = orc.OracleScene(scenes.load(name), width=w, height=h)
+    if precision == "direct":  # the direct integrator (one kernel per chunk)
+        o = orc.OracleScene(scenes.load(name), width=w, height=h, integrator="direct")
+        precision = rtg.PRECISION_F32
+    else:
+        o = orc.OracleScene(scenes.load(name), width=w, height=h)
     cfg = o.settings(precision)
     passes = [(0, 1), (1, 2), (2, 4), (4, 8)]
     monkeypatch.setenv("RTG_OVERLAP", "0")
