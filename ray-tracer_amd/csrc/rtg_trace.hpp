@@ -406,7 +406,7 @@ constexpr uint32_t kNoRef = 0xFFFFFFFFu;
 // would depend on how paths are grouped into waves. With a rule of the lane's
 // own, images and counters are independent of chunking, batching and queue
 // order.
-constexpr uint32_t kDescent = 16;
+constexpr uint32_t kDescent = 24;
 struct Walk {
   uint32_t ref;   // next node to visit (kNoRef: traversal done)
   uint32_t leaf;  // parked leaf (kNoRef: none)
