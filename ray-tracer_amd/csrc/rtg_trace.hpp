@@ -17,6 +17,8 @@
 // parks leaves (see Walk), so a ray may test a leaf after nodes that follow
 // it in depth-first order; every such choice depends on the ray alone.
 #pragma once
+#include <stddef.h>
+
 #include "rtg_math.hpp"
 #include "rtg_scene.hpp"
 
@@ -92,10 +94,9 @@ __device__ __forceinline__ bool sphere_hit64(V3<double> c, double radius, V3<dou
 }
 
 // shapes.hpp:110-127
-__device__ __forceinline__ bool tri_hit64(const rtg_primitive& p, V3<double> o, V3<double> d, double tmin,
-                                          double tmax, double& t_out, double& b1o, double& b2o) {
-  const V3<double> v0 = d3(p.v0);
-  const V3<double> e1 = d3(p.v1) - v0, e2 = d3(p.v2) - v0;
+__device__ __forceinline__ bool tri_hit64(V3<double> v0, V3<double> v1, V3<double> v2, V3<double> o, V3<double> d,
+                                          double tmin, double tmax, double& t_out, double& b1o, double& b2o) {
+  const V3<double> e1 = v1 - v0, e2 = v2 - v0;
   const V3<double> pvec = cross(d, e2);
   const double det = dot(e1, pvec);
   if (r_fabs(det) < 1e-14) return false;
@@ -114,10 +115,33 @@ __device__ __forceinline__ bool tri_hit64(const rtg_primitive& p, V3<double> o, 
   return true;
 }
 
-__device__ __forceinline__ bool prim_hit64(const rtg_primitive& p, V3<double> o, V3<double> d, double tmin,
-                                           double tmax, double& t, double& b1, double& b2) {
-  if (p.kind == RTG_PRIM_SPHERE) return sphere_hit64(d3(p.v0), p.v1.x, o, d, tmin, tmax, t);
-  return tri_hit64(p, o, d, tmin, tmax, t, b1, b2);
+// A primitive's kind and vertices (the first 88 bytes of rtg_primitive),
+// loaded together. Read field by field, the vertex loads would be issued
+// after the kind test, a second wait for memory per primitive. The array's
+// elements are 160 bytes apart from a 256-byte aligned base, so the 16-byte
+// loads are aligned; the empty asm statement keeps the last load ahead of
+// the branch.
+struct Prim64 {
+  uint32_t kind;
+  V3<double> v0, v1, v2;
+};
+static_assert(sizeof(rtg_primitive) == 160 && offsetof(rtg_primitive, v0) == 16 && offsetof(rtg_primitive, v1) == 40 &&
+                  offsetof(rtg_primitive, v2) == 64,
+              "load_prim64 reads the documented rtg_primitive layout");
+typedef double NativeD2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ Prim64 load_prim64(const rtg_primitive* p) {
+  const auto* q = (const __attribute__((address_space(1))) NativeD2*)p;
+  const uint32_t kind = *(const __attribute__((address_space(1))) uint32_t*)p;
+  const NativeD2 a = q[1], b = q[2], c = q[3], e = q[4];
+  const double f = ((const __attribute__((address_space(1))) double*)p)[10];
+  asm volatile("" ::"v"(f));
+  return Prim64{kind, {a.x, a.y, b.x}, {b.y, c.x, c.y}, {e.x, e.y, f}};
+}
+
+__device__ __forceinline__ bool prim_hit64(const Prim64& p, V3<double> o, V3<double> d, double tmin, double tmax,
+                                           double& t, double& b1, double& b2) {
+  if (p.kind == RTG_PRIM_SPHERE) return sphere_hit64(p.v0, p.v1.x, o, d, tmin, tmax, t);
+  return tri_hit64(p.v0, p.v1, p.v2, o, d, tmin, tmax, t, b1, b2);
 }
 
 // shapes.hpp:82-86 (t only); returns the plane index hit last (planes loop
@@ -155,7 +179,7 @@ __device__ inline Hit64 closest64(const DevScene& S, bool use_bvh, V3<double> o,
     for (uint32_t i = 0; i < S.n_prims; ++i) {
       ++c.tests;
       double th, b1, b2;
-      if (prim_hit64(S.prims[i], o, d, tmin, h.t, th, b1, b2)) {
+      if (prim_hit64(load_prim64(S.prims + i), o, d, tmin, h.t, th, b1, b2)) {
         h.t = th;
         h.b1 = b1;
         h.b2 = b2;
@@ -176,17 +200,21 @@ __device__ inline Hit64 closest64(const DevScene& S, bool use_bvh, V3<double> o,
     ++c.visits;
     if (box64(n, o, inv, tmin, tm)) {
       if (n.count > 0) {
+        // The next primitive's index is loaded before the current one is
+        // tested (the last reloads its own).
+        uint32_t pi = S.prim_indices[n.offset];
         for (uint32_t i = 0; i < n.count; ++i) {
+          const uint32_t pn = S.prim_indices[n.offset + (i + 1 < n.count ? i + 1 : i)];
           ++c.tests;
-          const uint32_t pi = S.prim_indices[n.offset + i];
           double th, b1, b2;
-          if (prim_hit64(S.prims[pi], o, d, tmin, tm, th, b1, b2)) {
+          if (prim_hit64(load_prim64(S.prims + pi), o, d, tmin, tm, th, b1, b2)) {
             h.t = th;
             h.b1 = b1;
             h.b2 = b2;
             h.prim = pi;
             tm = th;
           }
+          pi = pn;
         }
       } else {
         if (neg[n.axis]) {
@@ -220,7 +248,7 @@ __device__ inline bool any64(const DevScene& S, bool use_bvh, V3<double> o, V3<d
   if (!use_bvh) {
     for (uint32_t i = 0; i < S.n_prims; ++i) {
       ++c.tests;
-      if (prim_hit64(S.prims[i], o, d, tmin, tmax, th, b1, b2)) return true;
+      if (prim_hit64(load_prim64(S.prims + i), o, d, tmin, tmax, th, b1, b2)) return true;
     }
     return false;
   }
@@ -233,9 +261,12 @@ __device__ inline bool any64(const DevScene& S, bool use_bvh, V3<double> o, V3<d
     ++c.visits;
     if (box64(n, o, inv, tmin, tmax)) {
       if (n.count > 0) {
+        uint32_t pi = S.prim_indices[n.offset];
         for (uint32_t i = 0; i < n.count; ++i) {
+          const uint32_t pn = S.prim_indices[n.offset + (i + 1 < n.count ? i + 1 : i)];
           ++c.tests;
-          if (prim_hit64(S.prims[S.prim_indices[n.offset + i]], o, d, tmin, tmax, th, b1, b2)) return true;
+          if (prim_hit64(load_prim64(S.prims + pi), o, d, tmin, tmax, th, b1, b2)) return true;
+          pi = pn;
         }
       } else {
         stk.ref[sp++ * kStackStride] = n.offset;
