@@ -216,21 +216,20 @@ rtg_status validate_desc(rtg_context* ctx, const rtg_scene_desc* d) {
   return RTG_OK;
 }
 
-void profile_mark(void* user, int kind, bool begin, hipStream_t stream) {
+void profile_events(void* user, int kind, hipEvent_t* start, hipEvent_t* stop) {
   auto* ctx = static_cast<rtg_context*>(user);
-  if (ctx->events_used == ctx->events.size()) {
+  while (ctx->events.size() < ctx->events_used + 2) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
     ctx->events.push_back(e);
   }
-  if (begin) {
-    if (ctx->event_kind.size() < ctx->events_used + 1) ctx->event_kind.resize(ctx->events_used + 1);
-    ctx->event_kind[ctx->events_used] = kind;
-  }
-  hipEventRecord(ctx->events[ctx->events_used++], stream);
+  if (ctx->event_kind.size() < ctx->events_used + 1) ctx->event_kind.resize(ctx->events_used + 1);
+  ctx->event_kind[ctx->events_used] = kind;
+  *start = ctx->events[ctx->events_used++];
+  *stop = ctx->events[ctx->events_used++];
 }
 
-// Adds every recorded begin/end pair to the totals. Called when the profile
+// Adds every recorded start/stop pair to the totals. Called when the profile
 // is read and when many events are pending, not after every call, so that
 // the host does not read event times while the next pass could be running.
 void profile_flush_events(rtg_context* ctx) {
@@ -270,7 +269,7 @@ const SideStream* side_of(rtg_context* ctx, bool lane2 = false) {
 const LaunchHooks* hooks_of(rtg_context* ctx) {
   if (!ctx->profiling) return nullptr;
   ctx->hooks.user = ctx;
-  ctx->hooks.mark = profile_mark;
+  ctx->hooks.events = profile_events;
   return &ctx->hooks;
 }
 
