@@ -266,8 +266,10 @@ rtg_status rtg_synchronize(rtg_context* ctx);
 
 /* ------------------------------------------------------------------------ */
 /* Per-kernel device timing of the render passes and traces issued since
- * profiling was switched on (HIP events around every launch, on the
- * context's stream). kind: 0 generate, 1 extend (closest hit), 2 shade,
+ * profiling was switched on (a pair of HIP events per launch, recorded by
+ * the kernel's dispatch at its start and end: hipExtLaunchKernel; batch
+ * passes launch directly instead of replaying graphs while profiling is
+ * on). kind: 0 generate, 1 extend (closest hit), 2 shade,
  * 3 shadow (any hit), 4 accumulate, 5 direct/normals sample, 6 trace,
  * 7 batch planning. */
 enum { RTG_K_GENERATE = 0, RTG_K_EXTEND, RTG_K_SHADE, RTG_K_SHADOW, RTG_K_ACCUMULATE, RTG_K_DIRECT, RTG_K_TRACE,
