@@ -282,7 +282,12 @@ typedef struct rtg_profile {
   uint64_t closest_rays, closest_visits, closest_tests;
   uint64_t shadow_rays, shadow_visits, shadow_tests;
 } rtg_profile;
-rtg_status rtg_set_profiling(rtg_context* ctx, int32_t on); /* on: clears the totals */
+rtg_status rtg_set_profiling(rtg_context* ctx, int32_t on); /* on: every kind; clears the totals */
+/* Times only the launches of the kinds in `kinds` (bit 1 << RTG_K_*; 0 =
+ * off; clears the totals). The work totals are kept either way. Fewer
+ * timed launches perturb a pass less: bench.py times the closest-hit
+ * launches only. No reference counterpart (a measurement hook). */
+rtg_status rtg_set_profiling_kinds(rtg_context* ctx, uint32_t kinds);
 rtg_status rtg_get_profile(rtg_context* ctx, rtg_profile* out);
 
 #ifdef __cplusplus

@@ -76,6 +76,7 @@ struct rtg_context {
   size_t rgb_cap = 0;
   // Profiling: an event pair per launch, resolved when the call completes.
   bool profiling = false;
+  uint32_t profile_kinds = 0;  // RTG_K_* kinds whose launches are timed
   std::vector<hipEvent_t> events;
   size_t events_used = 0;
   std::vector<int> event_kind;  // kind of each begin event
@@ -218,6 +219,7 @@ rtg_status validate_desc(rtg_context* ctx, const rtg_scene_desc* d) {
 
 void profile_events(void* user, int kind, hipEvent_t* start, hipEvent_t* stop) {
   auto* ctx = static_cast<rtg_context*>(user);
+  if (!(ctx->profile_kinds & (1u << kind))) return;
   while (ctx->events.size() < ctx->events_used + 2) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
@@ -281,7 +283,14 @@ int rtg_abi_version(void) { return RTG_ABI_VERSION; }
 
 rtg_status rtg_set_profiling(rtg_context* ctx, int32_t on) {
   if (!ctx) return fail(nullptr, RTG_ERR_INVALID_ARG, "rtg_set_profiling: NULL context");
-  ctx->profiling = on != 0;
+  return rtg_set_profiling_kinds(ctx, on ? (1u << RTG_K_COUNT) - 1u : 0u);
+}
+
+rtg_status rtg_set_profiling_kinds(rtg_context* ctx, uint32_t kinds) {
+  if (!ctx) return fail(nullptr, RTG_ERR_INVALID_ARG, "rtg_set_profiling_kinds: NULL context");
+  if (kinds >> RTG_K_COUNT) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_set_profiling_kinds: unknown kernel kind");
+  ctx->profiling = kinds != 0;
+  ctx->profile_kinds = kinds;
   ctx->profile = rtg_profile{};
   ctx->events_used = 0;
   return RTG_OK;

@@ -168,7 +168,7 @@ def rtg_symbols():
             "rtg_accum_read", "rtg_accum_min_samples", "rtg_render_pass", "rtg_render_batch", "rtg_resolve",
             "rtg_buffer_create",
             "rtg_buffer_release", "rtg_buffer_write", "rtg_buffer_read", "rtg_trace", "rtg_trace_host",
-            "rtg_synchronize", "rtg_set_profiling", "rtg_get_profile"]
+            "rtg_synchronize", "rtg_set_profiling", "rtg_set_profiling_kinds", "rtg_get_profile"]
 
 
 def _ptr(a):
@@ -539,10 +539,19 @@ class Profile(C.Structure):
         return d
 
 
-def set_profiling(ctx: "Context", on: bool):
+def set_profiling(ctx: "Context", on: bool, kinds=None):
+    """Per-launch timing on/off; `kinds` (names of K_NAMES) limits it to
+    those kernels (rtg_set_profiling_kinds)."""
     lib = ctx.lib
-    lib.rtg_set_profiling.restype, lib.rtg_set_profiling.argtypes = C.c_int32, [_VP, C.c_int32]
-    ctx._check(lib.rtg_set_profiling(ctx.h, 1 if on else 0))
+    if kinds is None:
+        lib.rtg_set_profiling.restype, lib.rtg_set_profiling.argtypes = C.c_int32, [_VP, C.c_int32]
+        ctx._check(lib.rtg_set_profiling(ctx.h, 1 if on else 0))
+        return
+    mask = 0
+    for k in kinds:
+        mask |= 1 << K_NAMES.index(k)
+    lib.rtg_set_profiling_kinds.restype, lib.rtg_set_profiling_kinds.argtypes = C.c_int32, [_VP, C.c_uint32]
+    ctx._check(lib.rtg_set_profiling_kinds(ctx.h, mask if on else 0))
 
 
 def get_profile(ctx: "Context") -> Profile:

@@ -15,10 +15,10 @@ COUNTERS = ("camera_samples", "rays", "shadow_rays", "rr_terminated", "invalid_s
 PASSES = [(0, 1), (1, 2), (2, 4)]
 
 
-def render(ctx, o, cfg, profile):
+def render(ctx, o, cfg, profile, kinds=None):
     dev = ctx.scene(o.desc)
     acc = ctx.accum(cfg.width, cfg.height)
-    rtg.set_profiling(ctx, profile)
+    rtg.set_profiling(ctx, profile, kinds=kinds)
     stats = [ctx.render_pass(dev, cfg, s0, s1, acc)[0] for s0, s1 in PASSES]
     prof = rtg.get_profile(ctx).as_dict()
     rtg.set_profiling(ctx, False)
@@ -86,3 +86,22 @@ def test_profiled_batch_is_unchanged_and_counted(ctx):
     assert prof["plan"]["launches"] == depth + 1
     assert prof["closest_rays"] == sum(a.rays for a in st1)
     assert prof["shadow_rays"] == sum(a.shadow_rays for a in st1)
+
+
+def test_profiling_selected_kinds(ctx):
+    """rtg_set_profiling_kinds: only the chosen kernels' launches are timed
+    (bench.py times the closest-hit kernel alone); the work totals are kept
+    and the render is unchanged."""
+    o = orc.OracleScene(scenes.load("torus_c2.json"), width=96, height=64)
+    cfg = o.settings(rtg.PRECISION_F32)
+    s0, c0, st0, _ = render(ctx, o, cfg, False)
+    s1, c1, st1, prof = render(ctx, o, cfg, True, kinds=["extend"])
+    assert np.array_equal(c1, c0)
+    assert np.array_equal(s1.view(np.uint64), s0.view(np.uint64))
+    assert prof["extend"]["launches"] == len(PASSES) * cfg.max_depth and prof["extend"]["ms"] > 0.0
+    for k in ("generate", "shade", "shadow", "accumulate", "direct", "trace", "plan"):
+        assert prof[k]["launches"] == 0 and prof[k]["ms"] == 0.0, k
+    assert prof["closest_rays"] == sum(a.rays for a in st1)
+    assert prof["shadow_rays"] == sum(a.shadow_rays for a in st1)
+    with pytest.raises(rtg.RtgError, match="unknown kernel kind"):
+        ctx._check(ctx.lib.rtg_set_profiling_kinds(ctx.h, 1 << len(rtg.K_NAMES)))
