@@ -594,6 +594,7 @@ __device__ inline bool any32(const DevScene& S, bool use_bvh, const Ray32& r, fl
   ++c.visits;
   if (!box32(r, S.root_lo[0], S.root_lo[1], S.root_lo[2], S.root_hi[0], S.root_hi[1], S.root_hi[2], tmin, tmax, tn))
     return false;
+  const uint32_t neg = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
   Walk w{S.root_ref, kNoRef, 0, 0};
   w.park(stk);
   for (;;) {
@@ -606,7 +607,12 @@ __device__ inline bool any32(const DevScene& S, bool use_bvh, const Ray32& r, fl
       const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, tmax, tl);
       const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, tmax, tr);
       c.visits += 2;
-      w.step(stk, under, m.x, m.y, hl, hr);  // Bvh::any visits idx+1 (left) first, pushes the right child
+      // Any order finds whether some hit lies in (tmin, tmax). Bvh::any
+      // visits the left child first (bvh.hpp:106-132); the far child first
+      // (by the direction's sign on the split axis) is 1.6 % faster here,
+      // near-first 1.4 % slower (torus workload, any-hit kernel alone).
+      const bool far_right = !((neg >> m.z) & 1u);
+      w.step(stk, under, far_right ? m.y : m.x, far_right ? m.x : m.y, far_right ? hr : hl, far_right ? hl : hr);
       if (w.descended(steps)) break;
     }
     w.park(stk);  // a leaf popped after a miss
