@@ -95,7 +95,7 @@ def cpu_baseline(scene_text, width, height, spp_full, target_s=12.0):
 def extend_roofline(prof, traffic):
     """Roofline of the closest-hit kernel from a profile: algorithmic bytes per
     launch (node boxes, triangle tests, ray in + hit out) over its average
-    launch duration (HIP events around every launch on its stream)."""
+    launch duration (HIP events recorded by each launch's dispatch)."""
     ext = prof["extend"]
     n_launch = max(1, ext["launches"])
     bytes_total = (NODE_B * prof["closest_visits"] + TRI_B * prof["closest_tests"] +
@@ -319,6 +319,8 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = extend_roofline(prof, traffic)
+    node_bytes = NODE_B * (prof["closest_visits"] + prof["shadow_visits"])
+    node_rate = node_bytes / secs / 1e9
     roofline["concurrent"] = ("a pass runs as two chunks on two streams, each with its any-hit kernels on a "
                               "third or fourth stream, so a launch shares the GPU with up to three other kernels "
                               "and its duration includes that time; roofline_serial is the kernel alone")
@@ -366,6 +368,11 @@ def main():
                    "kernel_ms_per_step": kernel_ms},
         "roofline": roofline,
         "roofline_serial": roofline_serial,
+        # The whole render's BVH node traffic (32 B per node box tested, by
+        # the closest-hit and any-hit kernels together) over its wall time:
+        # the node-traffic rate the GPU sustains across concurrent kernels.
+        "node_traffic": {"bytes_per_step": node_bytes // args.steps, "achieved": node_rate, "unit": "GB/s",
+                         "peak": HBM_PEAK_GBS, "frac": node_rate / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "batched": batch,
         "served": served,
