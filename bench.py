@@ -45,15 +45,13 @@ def load_pkg():
 
 
 def render_once(rtg, ctx, dev, acc, cfg, spp, max_pass=32):
-    """One progressive render of `spp` samples (renderer.cpp:66-69)."""
+    """One progressive render of `spp` samples: the pass schedule of
+    renderer.cpp:66-69 in one rtg_render_passes call, as GpuRenderer::render
+    issues it when no per-pass callback reads the framebuffer."""
     acc.clear()
-    done, rays = 0, 0
-    while done < spp:
-        k = min(spp - done, max(1, done), max_pass)
-        st, _ = ctx.render_pass(dev, cfg, done, done + k, acc)
-        rays += st.rays + st.shadow_rays
-        done += k
-    return rays
+    bounds = [0] + [s1 for _, s1 in pass_schedule(spp, max_pass)]
+    st, _, _ = ctx.render_passes(dev, cfg, bounds, acc)
+    return st.rays + st.shadow_rays
 
 
 def cpu_baseline(scene_text, width, height, spp_full, target_s=12.0):

@@ -204,6 +204,21 @@ rtg_status rtg_render_pass(rtg_context* ctx, const rtg_scene* scene, const rtg_r
                            uint32_t s0, uint32_t s1, rtg_accum* acc, const volatile int32_t* cancel,
                            rtg_stats* stats);
 
+/* Several progressive passes in one call: passes [bounds[i], bounds[i+1])
+ * for i < n_passes (bounds increasing), the loop of Renderer::render
+ * (rendering/renderer.cpp:58-105) when no on_pass callback needs the
+ * framebuffer between passes. Bitwise the result of calling
+ * rtg_render_pass for each pass in order (every pixel's additions stay in
+ * sample order), but without a host synchronisation between passes: the
+ * chunks of consecutive passes run side by side on the context's two lanes.
+ * `stats` totals the passes. `cancel`, if non-NULL, is read before each
+ * chunk with at most two chunks in flight; a raised flag returns
+ * RTG_ERR_CANCELLED with whole chunks kept. *passes_done (may be NULL): the
+ * passes completed. */
+rtg_status rtg_render_passes(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* settings,
+                             const uint32_t* bounds, uint32_t n_passes, rtg_accum* acc,
+                             const volatile int32_t* cancel, uint32_t* passes_done, rtg_stats* stats);
+
 /* Tone map + sRGB encode on the device (rendering/tonemap.cpp:14-53):
  * writes width*height*3 bytes. tone_map: 0 clamp, 1 reinhard, 2 aces. */
 rtg_status rtg_resolve(rtg_context* ctx, const rtg_accum* acc, int32_t tone_map, double exposure, uint8_t* rgb);

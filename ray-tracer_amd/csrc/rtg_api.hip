@@ -55,6 +55,7 @@ struct rtg_context {
   PathBuffers<float> pb32b{};
   PathBuffers<double> pb64b{};
   hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t acc_done[2] = {nullptr, nullptr};  // each lane's latest accumulation
   PathBuffers<double> pb64{};
   PathBuffers<float> pbb{};
   uint32_t cap[5] = {0, 0, 0, 0, 0};
@@ -328,6 +329,8 @@ rtg_status rtg_open(int device, rtg_context** out) {
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side2.shadowed[1], hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->acc_done[0], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->acc_done[1], hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&c->stats, sizeof(unsigned long long) * kStatSlots * kStatStride);
   if (e == hipSuccess) e = hipHostMalloc(&c->stats_host, sizeof(unsigned long long) * kStatSlots * kStatStride);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
@@ -367,7 +370,8 @@ rtg_status rtg_close(rtg_context* ctx) {
       hipStreamSynchronize(st);
       hipStreamDestroy(st);
     }
-  for (hipEvent_t e : {ctx->side2.shaded, ctx->side2.shadowed[0], ctx->side2.shadowed[1], ctx->fork, ctx->join})
+  for (hipEvent_t e : {ctx->side2.shaded, ctx->side2.shadowed[0], ctx->side2.shadowed[1], ctx->fork, ctx->join,
+                       ctx->acc_done[0], ctx->acc_done[1]})
     if (e) hipEventDestroy(e);
   for (hipEvent_t e : ctx->events) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -860,84 +864,96 @@ bool two_lanes_enabled() {
   return !(e && e[0] == '1');
 }
 
-// Samples [s0, s1) of pixels [px0, px1), one chunk of whole pixels at a time.
-// Path-integrator chunks alternate between two lanes (streams, path
-// buffers); the chunks of one sample sub-range cover disjoint pixels, and
-// the lanes join before the next sub-range, so every pixel's additions stay
-// in sample order.
-rtg_status render_range(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* cfg, uint32_t s0,
-                        uint32_t s1, uint32_t px0, uint32_t px1, rtg_accum* acc, const volatile int32_t* cancel,
-                        rtg_stats* stats, const char* fn) {
+// Passes [bounds[i], bounds[i + 1]) of pixels [px0, px1), i < n_pass, in
+// order, with no host synchronisation between chunks or passes. Each pass is
+// cut into chunks of whole pixels x a sample sub-range; path-integrator
+// chunks alternate between two lanes (streams, path buffers), so a chunk's
+// kernels run beside the other lane's, across pass boundaries too (a small
+// pass fills the GPU poorly alone: its bounces' launches are short and end
+// in tails). A chunk's accumulation waits for the other lane's latest one
+// and each lane runs its chunks in order, so every pixel's additions stay in
+// sample order: the result is bitwise that of the passes one after another.
+// With `cancel`, at most two chunks are in flight: before each chunk the one
+// two back (the lane's previous) has finished and the flag is read. Returns
+// the passes completed in *passes_done.
+rtg_status render_ranges(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* cfg,
+                         const uint32_t* bounds, uint32_t n_pass, uint32_t px0, uint32_t px1, rtg_accum* acc,
+                         const volatile int32_t* cancel, uint32_t* passes_done, rtg_stats* stats, const char* fn) {
   const bool f64 = cfg->precision == RTG_PRECISION_F64;
-  const uint64_t paths = (uint64_t)(s1 - s0) * (px1 - px0);
-  // Small passes stay on one lane: splitting them only adds launches.
-  const bool two = cfg->integrator == RTG_INTEGRATOR_PATH &&
-                   paths >= std::min<uint64_t>(kTwoLanePaths, chunk_paths_default()) && two_lanes_enabled();
-  const uint32_t cap = capacity_for(two ? (paths + 1) / 2 : paths);
+  const bool lanes = cfg->integrator == RTG_INTEGRATOR_PATH && two_lanes_enabled();
+  const uint64_t split = std::min<uint64_t>(kTwoLanePaths, chunk_paths_default());
+  // A pass of at least `split` paths is cut into (at least) two chunks.
+  auto pass_cap = [&](uint32_t i) {
+    const uint64_t paths = (uint64_t)(bounds[i + 1] - bounds[i]) * (px1 - px0);
+    return capacity_for(lanes && paths >= split ? (paths + 1) / 2 : paths);
+  };
+  uint32_t cap_max = 1;
+  for (uint32_t i = 0; i < n_pass; ++i) cap_max = std::max(cap_max, pass_cap(i));
+  const bool two = lanes && (n_pass > 1 || (uint64_t)(bounds[1] - bounds[0]) * (px1 - px0) >= split);
   if (f64) {
-    if (rtg_status st = ensure_paths(ctx, 1, cap, ctx->pb64)) return st;
+    if (rtg_status st = ensure_paths(ctx, 1, cap_max, ctx->pb64)) return st;
     if (two)
-      if (rtg_status st = ensure_paths(ctx, kLane64Buffers, cap, ctx->pb64b)) return st;
+      if (rtg_status st = ensure_paths(ctx, kLane64Buffers, cap_max, ctx->pb64b)) return st;
   } else {
-    if (rtg_status st = ensure_paths(ctx, 0, cap, ctx->pb32)) return st;
+    if (rtg_status st = ensure_paths(ctx, 0, cap_max, ctx->pb32)) return st;
     if (two)
-      if (rtg_status st = ensure_paths(ctx, kLaneBuffers, cap, ctx->pb32b)) return st;
+      if (rtg_status st = ensure_paths(ctx, kLaneBuffers, cap_max, ctx->pb32b)) return st;
   }
   HIP_TRY(ctx, hipMemsetAsync(ctx->stats, 0, sizeof(unsigned long long) * kStatSlots * kStatStride, ctx->stream));
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-  auto fork = [&]() -> rtg_status {  // the second lane starts after the main stream's work so far
+  if (two) {  // the second lane starts after the main stream's work so far
     HIP_TRY(ctx, hipEventRecord(ctx->fork, ctx->stream));
     HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream2, ctx->fork, 0));
-    return RTG_OK;
-  };
-  auto join = [&]() -> rtg_status {  // the main stream continues after the second lane's work
+  }
+  bool used[2] = {false, false};
+  uint32_t lane = 0, done = 0;
+  bool cancelled = false;
+  for (uint32_t i = 0; i < n_pass && !cancelled; ++i) {
+    const uint32_t s0 = bounds[i], s1 = bounds[i + 1], k_total = s1 - s0, cap = pass_cap(i);
+    // Sample sub-ranges outermost keep every pixel's additions in sample order.
+    for (uint32_t sa = s0; sa < s1 && !cancelled;) {
+      const uint32_t k = std::min(k_total - (sa - s0), cap);
+      const uint32_t pix_per_chunk = std::max(1u, cap / k);
+      for (uint32_t p0 = px0; p0 < px1; p0 += pix_per_chunk) {
+        if (cancel) {
+          if (used[lane]) HIP_TRY(ctx, hipEventSynchronize(ctx->acc_done[lane]));
+          if (*cancel) {
+            cancelled = true;
+            break;
+          }
+        }
+        const bool lane2 = lane == 1;
+        const PathBuffers<float>& B32 = lane2 ? ctx->pb32b : ctx->pb32;
+        const PathBuffers<double>& B64 = lane2 ? ctx->pb64b : ctx->pb64;
+        ChunkParams P = render_params(cfg, acc);
+        P.p0 = p0;
+        P.k = k;
+        P.s0 = sa;
+        P.n_pix = std::min(pix_per_chunk, px1 - p0);
+        P.n_paths = P.n_pix * k;
+        P.sub_cap = f64 ? B64.sub_cap : B32.sub_cap;
+        P.shadow_counts = kMaxBounce * kSubQueues;
+        P.stat_slots = kStatSlots;
+        P.counts = f64 ? B64.counts : B32.counts;
+        P.stats = ctx->stats;
+        const hipStream_t st = lane2 ? ctx->stream2 : ctx->stream;
+        const hipEvent_t wait = used[lane ^ 1] ? ctx->acc_done[lane ^ 1] : nullptr;
+        const hipEvent_t rec = (two || cancel) ? ctx->acc_done[lane] : nullptr;
+        if (f64)
+          run_chunk(scene->dev, B64, P, scene->stack, st, side_of(ctx, lane2), hooks_of(ctx), wait, rec);
+        else
+          run_chunk(scene->dev, B32, P, scene->stack, st, side_of(ctx, lane2), hooks_of(ctx), wait, rec);
+        HIP_TRY(ctx, hipGetLastError());
+        used[lane] = true;
+        if (two) lane ^= 1;
+      }
+      sa += k;
+    }
+    if (!cancelled) ++done;
+  }
+  if (used[1]) {  // the main stream continues after the second lane's work
     HIP_TRY(ctx, hipEventRecord(ctx->join, ctx->stream2));
     HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->join, 0));
-    return RTG_OK;
-  };
-  const uint32_t k_total = s1 - s0;
-  bool cancelled = false;
-  // Sample sub-ranges outermost keep every pixel's additions in sample order.
-  for (uint32_t sa = s0; sa < s1 && !cancelled;) {
-    const uint32_t k = std::min(k_total - (sa - s0), cap);
-    const uint32_t pix_per_chunk = std::max(1u, cap / k);
-    if (two)
-      if (rtg_status st = fork()) return st;
-    uint32_t chunk = 0;
-    for (uint32_t p0 = px0; p0 < px1; p0 += pix_per_chunk, ++chunk) {
-      if (cancel && *cancel) {
-        cancelled = true;
-        break;
-      }
-      const bool lane2 = two && (chunk & 1);
-      const PathBuffers<float>& B32 = lane2 ? ctx->pb32b : ctx->pb32;
-      const PathBuffers<double>& B64 = lane2 ? ctx->pb64b : ctx->pb64;
-      ChunkParams P = render_params(cfg, acc);
-      P.p0 = p0;
-      P.k = k;
-      P.s0 = sa;
-      P.n_pix = std::min(pix_per_chunk, px1 - p0);
-      P.n_paths = P.n_pix * k;
-      P.sub_cap = f64 ? B64.sub_cap : B32.sub_cap;
-      P.shadow_counts = kMaxBounce * kSubQueues;
-      P.stat_slots = kStatSlots;
-      P.counts = f64 ? B64.counts : B32.counts;
-      P.stats = ctx->stats;
-      if (f64)
-        run_chunk(scene->dev, B64, P, scene->stack, lane2 ? ctx->stream2 : ctx->stream, side_of(ctx, lane2),
-                  hooks_of(ctx));
-      else
-        run_chunk(scene->dev, B32, P, scene->stack, lane2 ? ctx->stream2 : ctx->stream,
-                  side_of(ctx, lane2), hooks_of(ctx));
-      HIP_TRY(ctx, hipGetLastError());
-      if (cancel) {  // poll between chunks
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        if (two) HIP_TRY(ctx, hipStreamSynchronize(ctx->stream2));
-      }
-    }
-    if (two)
-      if (rtg_status st = join()) return st;
-    sa += k;
   }
   if (rtg_status st = stage_stats(ctx)) return st;
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -945,8 +961,16 @@ rtg_status render_range(rtg_context* ctx, const rtg_scene* scene, const rtg_rend
   float ms = 0;
   hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
   read_stats(ctx, stats, ms);
+  if (passes_done) *passes_done = done;
   if (cancelled) return fail(ctx, RTG_ERR_CANCELLED, std::string(fn) + ": cancelled");
   return RTG_OK;
+}
+
+rtg_status render_range(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* cfg, uint32_t s0,
+                        uint32_t s1, uint32_t px0, uint32_t px1, rtg_accum* acc, const volatile int32_t* cancel,
+                        rtg_stats* stats, const char* fn) {
+  const uint32_t bounds[2] = {s0, s1};
+  return render_ranges(ctx, scene, cfg, bounds, 1, px0, px1, acc, cancel, nullptr, stats, fn);
 }
 
 // ---------------------------------------------------------------------------
@@ -1103,6 +1127,20 @@ rtg_status rtg_render_pass(rtg_context* ctx, const rtg_scene* scene, const rtg_r
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   return render_range(ctx, scene, cfg, s0, s1, 0, (uint32_t)acc->w * (uint32_t)acc->h, acc, cancel, stats,
                       "rtg_render_pass");
+}
+
+rtg_status rtg_render_passes(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* cfg,
+                             const uint32_t* bounds, uint32_t n_passes, rtg_accum* acc, const volatile int32_t* cancel,
+                             uint32_t* passes_done, rtg_stats* stats) {
+  if (passes_done) *passes_done = 0;
+  if (!ctx || !scene || !cfg || !acc || (n_passes && !bounds))
+    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_passes: NULL argument");
+  if (n_passes == 0) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_passes: no passes");
+  for (uint32_t i = 0; i < n_passes; ++i)
+    if (rtg_status st = check_settings(ctx, "rtg_render_passes", cfg, acc, bounds[i], bounds[i + 1])) return st;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  return render_ranges(ctx, scene, cfg, bounds, n_passes, 0, (uint32_t)acc->w * (uint32_t)acc->h, acc, cancel,
+                       passes_done, stats, "rtg_render_passes");
 }
 
 rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jobs, rtg_stats* stats) {

@@ -11,7 +11,7 @@ inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBloc
 
 template <int STACK>
 void chunk_impl(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P, hipStream_t s,
-                const SideStream* side, const LaunchHooks* h) {
+                const SideStream* side, const LaunchHooks* h, hipEvent_t acc_wait, hipEvent_t acc_done) {
   const uint32_t g = grid_of(P.n_paths);
   if (P.integrator != RTG_INTEGRATOR_PATH) {
     RTG_LAUNCH(h, RTG_K_DIRECT, (k_direct<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P);
@@ -45,7 +45,9 @@ void chunk_impl(const DevScene& S, const PathBuffers<float>& B, const ChunkParam
     for (int p = 0; p < 2; ++p)
       if (pending[p]) hipStreamWaitEvent(s, side->shadowed[p], 0);
   }
+  if (acc_wait) hipStreamWaitEvent(s, acc_wait, 0);
   RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_accumulate<float>, dim3(accumulate_blocks<float>(P)), dim3(kBlock), 0, s, B, P);
+  if (acc_done) hipEventRecord(acc_done, s);
 }
 
 template <int STACK>
@@ -98,11 +100,11 @@ void trace_impl(const DevScene& S, const double* rays, uint64_t n, int mode, int
 }  // namespace
 
 void run_chunk(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
-               const SideStream* side, const LaunchHooks* h) {
+               const SideStream* side, const LaunchHooks* h, hipEvent_t acc_wait, hipEvent_t acc_done) {
   switch (stack_class_of(stack)) {
-    case 0: chunk_impl<19>(S, B, P, s, side, h); break;
-    case 1: chunk_impl<31>(S, B, P, s, side, h); break;
-    default: chunk_impl<63>(S, B, P, s, side, h); break;
+    case 0: chunk_impl<19>(S, B, P, s, side, h, acc_wait, acc_done); break;
+    case 1: chunk_impl<31>(S, B, P, s, side, h, acc_wait, acc_done); break;
+    default: chunk_impl<63>(S, B, P, s, side, h, acc_wait, acc_done); break;
   }
 }
 

@@ -47,7 +47,7 @@ __global__ void k_resolve(const double* sum, const uint32_t* count, int n, int o
 namespace {
 template <int STACK>
 void chunk_impl(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, hipStream_t s,
-                const SideStream* side, const LaunchHooks* h) {
+                const SideStream* side, const LaunchHooks* h, hipEvent_t acc_wait, hipEvent_t acc_done) {
   const uint32_t g = grid_of(P.n_paths);
   if (P.integrator != RTG_INTEGRATOR_PATH) {
     RTG_LAUNCH(h, RTG_K_DIRECT, (k_direct<double, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P);
@@ -77,7 +77,9 @@ void chunk_impl(const DevScene& S, const PathBuffers<double>& B, const ChunkPara
     }
     if (pending) hipStreamWaitEvent(s, side->shadowed[0], 0);
   }
+  if (acc_wait) hipStreamWaitEvent(s, acc_wait, 0);
   RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_accumulate<double>, dim3(accumulate_blocks<double>(P)), dim3(kBlock), 0, s, B, P);
+  if (acc_done) hipEventRecord(acc_done, s);
 }
 
 template <int STACK>
@@ -92,11 +94,11 @@ void trace_impl(const DevScene& S, const double* rays, uint64_t n, int mode, int
 // so its stack, kept in LDS, needs the BVH depth in entries: the FP32 stack
 // classes serve it.
 void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
-               const SideStream* side, const LaunchHooks* h) {
+               const SideStream* side, const LaunchHooks* h, hipEvent_t acc_wait, hipEvent_t acc_done) {
   switch (stack_class_of(stack)) {
-    case 0: chunk_impl<19>(S, B, P, s, side, h); break;
-    case 1: chunk_impl<31>(S, B, P, s, side, h); break;
-    default: chunk_impl<63>(S, B, P, s, side, h); break;
+    case 0: chunk_impl<19>(S, B, P, s, side, h, acc_wait, acc_done); break;
+    case 1: chunk_impl<31>(S, B, P, s, side, h, acc_wait, acc_done); break;
+    default: chunk_impl<63>(S, B, P, s, side, h, acc_wait, acc_done); break;
   }
 }
 

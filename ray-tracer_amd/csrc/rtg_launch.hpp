@@ -53,12 +53,15 @@ struct SideStream {
 };
 
 // Runs one chunk of a render pass: generate, max_depth x (extend, shade,
-// shadow), accumulate; or the per-sample direct/normals kernel.
+// shadow), accumulate; or the per-sample direct/normals kernel. The
+// accumulation waits for `acc_wait` (if not null: another lane's latest
+// accumulation) and records `acc_done` (if not null).
 void run_chunk(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
-               const SideStream* side, const LaunchHooks* hooks);
+               const SideStream* side, const LaunchHooks* hooks, hipEvent_t acc_wait = nullptr,
+               hipEvent_t acc_done = nullptr);
 void run_chunk(const DevScene& S, const PathBuffers<double>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,
-               const SideStream* side,
-               const LaunchHooks* hooks);
+               const SideStream* side, const LaunchHooks* hooks, hipEvent_t acc_wait = nullptr,
+               hipEvent_t acc_done = nullptr);
 
 // Grid sizes of one batched pass (persistent workgroups; see k_batch_*).
 struct BatchGrids {

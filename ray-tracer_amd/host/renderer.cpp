@@ -76,6 +76,19 @@ std::string GpuRenderer::device_name() const {
 
 uint64_t GpuRenderer::device_bytes() const { return rtg_scene_bytes(dscene_); }
 
+void GpuRenderer::add_stats(const rtg_stats& st, uint32_t passes) {
+  stats_.camera_samples += st.camera_samples;
+  stats_.rays += st.rays;
+  stats_.shadow_rays += st.shadow_rays;
+  stats_.rr_terminated += st.rr_terminated;
+  stats_.invalid_samples += st.invalid_samples;
+  stats_.node_visits += st.node_visits;
+  stats_.prim_tests += st.prim_tests;
+  stats_.device_ms += st.elapsed_ms;
+  stats_.passes += passes;
+  fb_.valid_ = false;
+}
+
 RenderProgress GpuRenderer::render(const ProgressFn& on_pass, const std::atomic<int32_t>* cancel) {
   const auto t0 = std::chrono::steady_clock::now();
   const uint32_t target = (uint32_t)scene_.settings.spp;
@@ -84,6 +97,36 @@ RenderProgress GpuRenderer::render(const ProgressFn& on_pass, const std::atomic<
   RenderProgress prog;
   prog.spp_target = scene_.settings.spp;
   uint32_t done = 0;
+  if (!on_pass) {
+    // Nothing reads the framebuffer between passes: the whole schedule in
+    // one call, whose passes run side by side on the device (bitwise the
+    // same result as the loop below).
+    std::vector<uint32_t> bounds{0};
+    while (bounds.back() < target) {
+      uint32_t k = target - bounds.back();
+      if (opts_.progressive)
+        k = std::min({k, std::max<uint32_t>(1, bounds.back()), (uint32_t)std::max(1, opts_.max_pass_spp)});
+      bounds.push_back(bounds.back() + k);
+    }
+    rtg_stats st{};
+    uint32_t passes = 0;
+    const rtg_status rc = rtg_render_passes(ctx_, dscene_, &cfg, bounds.data(), (uint32_t)bounds.size() - 1, acc_,
+                                            flag, &passes, &st);
+    if (rc != RTG_OK && rc != RTG_ERR_CANCELLED) rtg_check(rc, ctx_, "rtg_render_passes");
+    add_stats(st, passes);
+    prog.pass = (int)passes;
+    prog.elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats_.elapsed_ms = prog.elapsed_ms;
+    prog.rays = stats_.rays + stats_.shadow_rays;
+    if (rc == RTG_ERR_CANCELLED || (cancel && cancel->load())) {
+      prog.cancelled = true;
+      prog.spp_done = rtg_accum_min_samples(ctx_, acc_);
+      return prog;
+    }
+    prog.spp_done = target;
+    prog.finished = true;
+    return prog;
+  }
   while (done < target) {
     uint32_t k = target - done;
     if (opts_.progressive)
@@ -91,16 +134,7 @@ RenderProgress GpuRenderer::render(const ProgressFn& on_pass, const std::atomic<
     rtg_stats st{};
     const rtg_status rc = rtg_render_pass(ctx_, dscene_, &cfg, done, done + k, acc_, flag, &st);
     if (rc != RTG_OK && rc != RTG_ERR_CANCELLED) rtg_check(rc, ctx_, "rtg_render_pass");
-    stats_.camera_samples += st.camera_samples;
-    stats_.rays += st.rays;
-    stats_.shadow_rays += st.shadow_rays;
-    stats_.rr_terminated += st.rr_terminated;
-    stats_.invalid_samples += st.invalid_samples;
-    stats_.node_visits += st.node_visits;
-    stats_.prim_tests += st.prim_tests;
-    stats_.device_ms += st.elapsed_ms;
-    stats_.passes++;
-    fb_.valid_ = false;
+    add_stats(st, 1);
     prog.pass++;
     prog.elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     stats_.elapsed_ms = prog.elapsed_ms;

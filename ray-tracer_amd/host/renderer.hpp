@@ -4,7 +4,9 @@
 //
 //   * progressive passes of k = min(remaining, max(1, done), max_pass) samples
 //     (renderer.cpp:66-69): 1, 1, 2, 4, 8, ... — same pass boundaries;
-//   * on_pass runs on the calling thread between passes;
+//   * on_pass runs on the calling thread between passes; without on_pass the
+//     whole schedule is one rtg_render_passes call, whose passes overlap on
+//     the device (bitwise the same framebuffer);
 //   * cancel is polled between device chunks; a cancelled render reports
 //     spp_done = the minimum per-pixel sample count;
 //   * statistics use the reference's counting rules (rays = closest-hit
@@ -94,6 +96,7 @@ class GpuRenderer {
 
  private:
   friend class Framebuffer;
+  void add_stats(const rtg_stats& st, uint32_t passes);
   const CompiledScene& scene_;
   RenderOptions opts_;
   rtg_context* ctx_ = nullptr;

@@ -144,6 +144,8 @@ def load_rtg():
         "rtg_accum_read": (C.c_int32, [_VP, _VP, _VP, _VP]),
         "rtg_accum_min_samples": (C.c_uint32, [_VP, _VP]),
         "rtg_render_pass": (C.c_int32, [_VP, _VP, P(RenderSettings), C.c_uint32, C.c_uint32, _VP, _VP, P(Stats)]),
+        "rtg_render_passes": (C.c_int32, [_VP, _VP, P(RenderSettings), P(C.c_uint32), C.c_uint32, _VP, _VP,
+                                          P(C.c_uint32), P(Stats)]),
         "rtg_render_batch": (C.c_int32, [_VP, P(Job), C.c_uint32, P(Stats)]),
         "rtg_resolve": (C.c_int32, [_VP, _VP, C.c_int32, C.c_double, _VP]),
         "rtg_buffer_create": (C.c_int32, [_VP, C.c_uint64, P(_VP)]),
@@ -165,7 +167,8 @@ def rtg_symbols():
     """Names declared in include/rtg.h (checked by the CPU tests)."""
     return ["rtg_open", "rtg_close", "rtg_last_error", "rtg_abi_version", "rtg_device_info", "rtg_scene_create",
             "rtg_scene_release", "rtg_scene_with_camera", "rtg_scene_bytes", "rtg_accum_create", "rtg_accum_release", "rtg_accum_clear",
-            "rtg_accum_read", "rtg_accum_min_samples", "rtg_render_pass", "rtg_render_batch", "rtg_resolve",
+            "rtg_accum_read", "rtg_accum_min_samples", "rtg_render_pass", "rtg_render_passes", "rtg_render_batch",
+            "rtg_resolve",
             "rtg_buffer_create",
             "rtg_buffer_release", "rtg_buffer_write", "rtg_buffer_read", "rtg_trace", "rtg_trace_host",
             "rtg_synchronize", "rtg_set_profiling", "rtg_set_profiling_kinds", "rtg_get_profile"]
@@ -224,6 +227,20 @@ class Context:
             return st, True
         self._check(rc)
         return st, False
+
+    def render_passes(self, scene, settings: RenderSettings, bounds, acc, cancel=None):
+        """Passes [bounds[i], bounds[i + 1]) in one call (rtg_render_passes).
+        Returns (total Stats, passes completed, cancelled)."""
+        b = (C.c_uint32 * len(bounds))(*bounds)
+        st = Stats()
+        done = C.c_uint32(0)
+        cp = C.cast(C.pointer(cancel), C.c_void_p) if cancel is not None else None
+        rc = self.lib.rtg_render_passes(self.h, scene.h, C.byref(settings), b, len(bounds) - 1, acc.h, cp,
+                                        C.byref(done), C.byref(st))
+        if rc == -4:
+            return st, done.value, True
+        self._check(rc)
+        return st, done.value, False
 
     def render_batch(self, jobs):
         """jobs: list of (scene, accum, settings, s0, s1[, p0, p1]). Returns
