@@ -123,9 +123,11 @@ def pass_schedule(spp, max_pass=32):
 def batch_leg(rtg, seed=4, count=64):
     """The batched workload (scenes/make_batch.py): `count` complete renders
     from one seed, each with the reference's progressive pass schedule.
-    Plain: one render after another, each pass through rtg_render_pass.
+    Plain: one render after another, each pass through rtg_render_pass;
+    and one render after another, each in one rtg_render_passes call (what
+    GpuRenderer does when no callback reads the passes).
     Batched: pass i of every unfinished render in one rtg_render_batch call.
-    The images of both are compared bitwise."""
+    The images of all three are compared bitwise."""
     import hashlib
     sys.path.insert(0, os.path.join(PKG, "scenes"))
     import make_batch
@@ -161,6 +163,17 @@ def batch_leg(rtg, seed=4, count=64):
         a.clear()
     ctx.synchronize()
     t0 = time.perf_counter()
+    rays_q = 0
+    for d, a, c, sched in zip(devs, accs, cfgs, scheds):
+        st, _, _ = ctx.render_passes(d, c, [0] + [s1 for _, s1 in sched], a)
+        rays_q += st.rays + st.shadow_rays
+    ctx.synchronize()
+    passes_s = time.perf_counter() - t0
+    same_q = digests() == ref
+    for a in accs:
+        a.clear()
+    ctx.synchronize()
+    t0 = time.perf_counter()
     rays_b, calls = 0, 0
     for p in range(max(len(s) for s in scheds)):
         jobs = [(devs[i], accs[i], cfgs[i]) + scheds[i][p] for i in range(count) if p < len(scheds[i])]
@@ -180,9 +193,12 @@ def batch_leg(rtg, seed=4, count=64):
                         "; 64^2-1024^2, 1-256 spp, progressive passes",
             "paths": int(sum(c.width * c.height * int(h.info()["spp"]) for c, h in zip(cfgs, hs))),
             "rays": int(rays_b), "host_prep_s": round(prep_s, 2),
-            "plain_s": round(plain_s, 4), "batched_s": round(batched_s, 4), "batch_calls": calls,
-            "plain_g_rays_per_s": rays_p / plain_s / 1e9, "batched_g_rays_per_s": rays_b / batched_s / 1e9,
-            "speedup": plain_s / batched_s, "identical": identical and rays_p == rays_b}
+            "plain_s": round(plain_s, 4), "plain_passes_call_s": round(passes_s, 4),
+            "batched_s": round(batched_s, 4), "batch_calls": calls,
+            "plain_g_rays_per_s": rays_p / plain_s / 1e9, "plain_passes_call_g_rays_per_s": rays_q / passes_s / 1e9,
+            "batched_g_rays_per_s": rays_b / batched_s / 1e9,
+            "speedup": plain_s / batched_s, "speedup_vs_passes_call": passes_s / batched_s,
+            "identical": identical and same_q and rays_p == rays_b == rays_q}
 
 
 def serve_leg(rtg, qps=6.0, seconds=10.0, seed=5, burst=300, tick_paths=0):
