@@ -881,15 +881,20 @@ rtg_status render_ranges(rtg_context* ctx, const rtg_scene* scene, const rtg_ren
                          const volatile int32_t* cancel, uint32_t* passes_done, rtg_stats* stats, const char* fn) {
   const bool f64 = cfg->precision == RTG_PRECISION_F64;
   const bool lanes = cfg->integrator == RTG_INTEGRATOR_PATH && two_lanes_enabled();
-  const uint64_t split = std::min<uint64_t>(kTwoLanePaths, chunk_paths_default());
-  // A pass of at least `split` paths is cut into (at least) two chunks.
+  // A pass of at least `split` paths is cut into (at least) two chunks, one
+  // per lane. A pass followed by another shares the GPU with it anyway, so
+  // only the last is cut at kTwoLanePaths (measured on the torus: cutting
+  // the 16- or 4-spp passes too, 5.20 / 5.14 vs 5.24 G rays/s).
+  const uint64_t chunk = chunk_paths_default();
   auto pass_cap = [&](uint32_t i) {
     const uint64_t paths = (uint64_t)(bounds[i + 1] - bounds[i]) * (px1 - px0);
+    const uint64_t split = i + 1 < n_pass ? chunk : std::min<uint64_t>(kTwoLanePaths, chunk);
     return capacity_for(lanes && paths >= split ? (paths + 1) / 2 : paths);
   };
   uint32_t cap_max = 1;
   for (uint32_t i = 0; i < n_pass; ++i) cap_max = std::max(cap_max, pass_cap(i));
-  const bool two = lanes && (n_pass > 1 || (uint64_t)(bounds[1] - bounds[0]) * (px1 - px0) >= split);
+  const bool two =
+      lanes && (n_pass > 1 || (uint64_t)(bounds[1] - bounds[0]) * (px1 - px0) >= std::min<uint64_t>(kTwoLanePaths, chunk));
   if (f64) {
     if (rtg_status st = ensure_paths(ctx, 1, cap_max, ctx->pb64)) return st;
     if (two)
