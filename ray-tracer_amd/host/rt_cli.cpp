@@ -251,9 +251,12 @@ int main(int argc, char** argv) {
                 << " ms" << std::flush;
   };
 
+  // Without an observer of the passes (no stream, snapshots or progress
+  // lines) the renderer runs its schedule in one call, passes side by side.
+  const bool observed = stream || save_every || !snapshot_dir.empty() || !quiet;
   RenderProgress final;
   try {
-    final = renderer->render(on_pass, &g_cancel);
+    final = observed ? renderer->render(on_pass, &g_cancel) : renderer->render({}, &g_cancel);
   } catch (const std::exception& e) {
     if (stream) std::cout << "ERROR {\"message\":" << json_str(e.what()) << "}\n" << std::flush;
     std::cerr << "\nerror: " << e.what() << "\n";

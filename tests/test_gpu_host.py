@@ -59,6 +59,23 @@ def test_rt_cli_writes_png_and_ppm():
                 assert data.startswith(b"P6\n40 30\n255\n") and len(data) == len(b"P6\n40 30\n255\n") + 40 * 30 * 3
 
 
+def test_rt_cli_quiet_render_equals_observed_render():
+    """With --quiet and no stream the CLI renders its pass schedule in one
+    call (passes side by side); with progress output, pass by pass. The
+    images must be byte-identical."""
+    scene = os.path.join(scenes.SCENE_DIR, "torus_c2.json")
+    with tempfile.TemporaryDirectory() as d:
+        outs = []
+        for extra in (["-q"], []):
+            out = os.path.join(d, f"img{len(outs)}.ppm")
+            r = subprocess.run([RT, scene, "-o", out, "--width", "96", "--height", "64", "--spp", "16"] + extra,
+                               capture_output=True, timeout=120)
+            assert r.returncode == 0, r.stderr.decode()
+            outs.append(open(out, "rb").read())
+        assert outs[0] == outs[1]
+        assert b"pass 5: 16/16 spp" in r.stderr
+
+
 def test_rt_cli_stream_protocol():
     text = json.dumps(scenes.load("cornell_c1.json"))
     r = subprocess.run([RT, "-", "--stream", "--quiet", "--width", "32", "--height", "32", "--spp", "8"],
