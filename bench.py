@@ -335,9 +335,10 @@ def main():
     roofline = extend_roofline(prof, traffic)
     node_bytes = NODE_B * (prof["closest_visits"] + prof["shadow_visits"])
     node_rate = node_bytes / secs / 1e9
-    roofline["concurrent"] = ("a pass runs as two chunks on two streams, each with its any-hit kernels on a "
-                              "third or fourth stream, so a launch shares the GPU with up to three other kernels "
-                              "and its duration includes that time; roofline_serial is the kernel alone")
+    roofline["concurrent"] = ("the pass schedule runs in one call, its chunks alternating between two streams "
+                              "(consecutive passes side by side), each with its any-hit kernels on a third or "
+                              "fourth stream, so a launch shares the GPU with up to three other kernels and its "
+                              "duration includes that time; roofline_serial is the kernel alone")
     kernel_ms = {k: round(v["ms"] / args.steps, 3) for k, v in prof.items() if isinstance(v, dict) and v["launches"]}
     # The same kernel with every launch of a pass on one stream (not timed
     # for `value`): the traversal kernel's own rate.
