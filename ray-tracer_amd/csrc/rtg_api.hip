@@ -42,6 +42,11 @@ struct rtg_context {
   std::string err;
   unsigned long long* stats = nullptr;
   unsigned long long* stats_host = nullptr;  // pinned copy of `stats`, written at the end of a call
+  // Statistics slots of a batch's single-render jobs (one kStatSlots region
+  // per job), and their pinned copy.
+  unsigned long long* job_stats = nullptr;
+  unsigned long long* job_stats_host = nullptr;
+  uint32_t job_stats_cap = 0;
   int n_cu = 0;
   // Path state, grown on demand: FP32 chunks, FP64 chunks, FP32 batches.
   void* path_mem[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -352,6 +357,8 @@ rtg_status rtg_close(rtg_context* ctx) {
     if (p) hipFree(p);
   if (ctx->stats) hipFree(ctx->stats);
   if (ctx->stats_host) hipHostFree(ctx->stats_host);
+  if (ctx->job_stats) hipFree(ctx->job_stats);
+  if (ctx->job_stats_host) hipHostFree(ctx->job_stats_host);
   if (ctx->batch_jobs) hipFree(ctx->batch_jobs);
   if (ctx->batch_plans) hipFree(ctx->batch_plans);
   if (ctx->batch_host) hipHostFree(ctx->batch_host);
@@ -864,48 +871,62 @@ bool two_lanes_enabled() {
   return !(e && e[0] == '1');
 }
 
-// Passes [bounds[i], bounds[i + 1]) of pixels [px0, px1), i < n_pass, in
-// order, with no host synchronisation between chunks or passes. Each pass is
-// cut into chunks of whole pixels x a sample sub-range; path-integrator
-// chunks alternate between two lanes (streams, path buffers), so a chunk's
-// kernels run beside the other lane's, across pass boundaries too (a small
-// pass fills the GPU poorly alone: its bounces' launches are short and end
-// in tails). A chunk's accumulation waits for the other lane's latest one
-// and each lane runs its chunks in order, so every pixel's additions stay in
-// sample order: the result is bitwise that of the passes one after another.
-// With `cancel`, at most two chunks are in flight: before each chunk the one
-// two back (the lane's previous) has finished and the flag is read. Returns
-// the passes completed in *passes_done.
-rtg_status render_ranges(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* cfg,
-                         const uint32_t* bounds, uint32_t n_pass, uint32_t px0, uint32_t px1, rtg_accum* acc,
-                         const volatile int32_t* cancel, uint32_t* passes_done, rtg_stats* stats, const char* fn) {
-  const bool f64 = cfg->precision == RTG_PRECISION_F64;
-  const bool lanes = cfg->integrator == RTG_INTEGRATOR_PATH && two_lanes_enabled();
-  // A pass of at least `split` paths is cut into (at least) two chunks, one
-  // per lane. A pass followed by another shares the GPU with it anyway, so
-  // only the last is cut at kTwoLanePaths (measured on the torus: cutting
-  // the 16- or 4-spp passes too, 5.20 / 5.14 vs 5.24 G rays/s).
+// One pass (samples [s0, s1) of pixels [px0, px1)) of one render, and the
+// statistics slots (kStatSlots) its kernels add to.
+struct PassItem {
+  const rtg_scene* scene;
+  const rtg_render_settings* cfg;
+  rtg_accum* acc;
+  uint32_t s0, s1, px0, px1;
+  unsigned long long* stats;
+};
+
+// Runs `items` in order with no host synchronisation between chunks or
+// items (the caller clears the statistics slots before and reads them after).
+// Each item is cut into chunks of whole pixels x a sample sub-range;
+// path-integrator chunks alternate between two lanes (streams, path
+// buffers), so a chunk's kernels run beside the other lane's, across item
+// boundaries too (a small pass fills the GPU poorly alone: its bounces'
+// launches are short and end in tails). A chunk's accumulation waits for the
+// other lane's latest one and each lane runs its chunks in order, so every
+// pixel's additions stay in sample order: the result is bitwise that of the
+// items one after another. With `cancel`, at most two chunks are in flight:
+// before each chunk the lane's previous one has finished and the flag is
+// read. Returns the items completed in *done_out.
+rtg_status run_items(rtg_context* ctx, const PassItem* items, uint32_t n, const volatile int32_t* cancel,
+                     uint32_t* done_out) {
   const uint64_t chunk = chunk_paths_default();
-  auto pass_cap = [&](uint32_t i) {
-    const uint64_t paths = (uint64_t)(bounds[i + 1] - bounds[i]) * (px1 - px0);
-    const uint64_t split = i + 1 < n_pass ? chunk : std::min<uint64_t>(kTwoLanePaths, chunk);
-    return capacity_for(lanes && paths >= split ? (paths + 1) / 2 : paths);
+  const uint64_t split_last = std::min<uint64_t>(kTwoLanePaths, chunk);
+  auto lanes_of = [&](uint32_t i) {
+    return items[i].cfg->integrator == RTG_INTEGRATOR_PATH && two_lanes_enabled();
   };
-  uint32_t cap_max = 1;
-  for (uint32_t i = 0; i < n_pass; ++i) cap_max = std::max(cap_max, pass_cap(i));
-  const bool two =
-      lanes && (n_pass > 1 || (uint64_t)(bounds[1] - bounds[0]) * (px1 - px0) >= std::min<uint64_t>(kTwoLanePaths, chunk));
-  if (f64) {
-    if (rtg_status st = ensure_paths(ctx, 1, cap_max, ctx->pb64)) return st;
-    if (two)
-      if (rtg_status st = ensure_paths(ctx, kLane64Buffers, cap_max, ctx->pb64b)) return st;
-  } else {
-    if (rtg_status st = ensure_paths(ctx, 0, cap_max, ctx->pb32)) return st;
-    if (two)
-      if (rtg_status st = ensure_paths(ctx, kLaneBuffers, cap_max, ctx->pb32b)) return st;
+  auto paths_of = [&](uint32_t i) { return (uint64_t)(items[i].s1 - items[i].s0) * (items[i].px1 - items[i].px0); };
+  // An item of at least `split` paths is cut into (at least) two chunks, one
+  // per lane. An item followed by another shares the GPU with it anyway, so
+  // only the last is cut from kTwoLanePaths on (measured on the torus pass
+  // schedule: cutting the 16- or 4-spp passes too, 5.20 / 5.14 vs 5.24 G
+  // rays/s).
+  auto cap_of = [&](uint32_t i) {
+    const uint64_t paths = paths_of(i);
+    return capacity_for(lanes_of(i) && paths >= (i + 1 < n ? chunk : split_last) ? (paths + 1) / 2 : paths);
+  };
+  uint32_t cap32 = 0, cap64 = 0;
+  bool two = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t& c = items[i].cfg->precision == RTG_PRECISION_F64 ? cap64 : cap32;
+    c = std::max(c, cap_of(i));
+    two = two || (lanes_of(i) && (n > 1 || paths_of(i) >= split_last));
   }
-  HIP_TRY(ctx, hipMemsetAsync(ctx->stats, 0, sizeof(unsigned long long) * kStatSlots * kStatStride, ctx->stream));
-  HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if (cap32) {
+    if (rtg_status st = ensure_paths(ctx, 0, cap32, ctx->pb32)) return st;
+    if (two)
+      if (rtg_status st = ensure_paths(ctx, kLaneBuffers, cap32, ctx->pb32b)) return st;
+  }
+  if (cap64) {
+    if (rtg_status st = ensure_paths(ctx, 1, cap64, ctx->pb64)) return st;
+    if (two)
+      if (rtg_status st = ensure_paths(ctx, kLane64Buffers, cap64, ctx->pb64b)) return st;
+  }
   if (two) {  // the second lane starts after the main stream's work so far
     HIP_TRY(ctx, hipEventRecord(ctx->fork, ctx->stream));
     HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream2, ctx->fork, 0));
@@ -913,13 +934,17 @@ rtg_status render_ranges(rtg_context* ctx, const rtg_scene* scene, const rtg_ren
   bool used[2] = {false, false};
   uint32_t lane = 0, done = 0;
   bool cancelled = false;
-  for (uint32_t i = 0; i < n_pass && !cancelled; ++i) {
-    const uint32_t s0 = bounds[i], s1 = bounds[i + 1], k_total = s1 - s0, cap = pass_cap(i);
+  for (uint32_t i = 0; i < n && !cancelled; ++i) {
+    const PassItem& it = items[i];
+    const bool f64 = it.cfg->precision == RTG_PRECISION_F64;
+    const bool alt = two && lanes_of(i);
+    const uint32_t k_total = it.s1 - it.s0, cap = cap_of(i);
     // Sample sub-ranges outermost keep every pixel's additions in sample order.
-    for (uint32_t sa = s0; sa < s1 && !cancelled;) {
-      const uint32_t k = std::min(k_total - (sa - s0), cap);
+    for (uint32_t sa = it.s0; sa < it.s1 && !cancelled;) {
+      const uint32_t k = std::min(k_total - (sa - it.s0), cap);
       const uint32_t pix_per_chunk = std::max(1u, cap / k);
-      for (uint32_t p0 = px0; p0 < px1; p0 += pix_per_chunk) {
+      for (uint32_t p0 = it.px0; p0 < it.px1; p0 += pix_per_chunk) {
+        if (!alt) lane = 0;
         if (cancel) {
           if (used[lane]) HIP_TRY(ctx, hipEventSynchronize(ctx->acc_done[lane]));
           if (*cancel) {
@@ -930,27 +955,27 @@ rtg_status render_ranges(rtg_context* ctx, const rtg_scene* scene, const rtg_ren
         const bool lane2 = lane == 1;
         const PathBuffers<float>& B32 = lane2 ? ctx->pb32b : ctx->pb32;
         const PathBuffers<double>& B64 = lane2 ? ctx->pb64b : ctx->pb64;
-        ChunkParams P = render_params(cfg, acc);
+        ChunkParams P = render_params(it.cfg, it.acc);
         P.p0 = p0;
         P.k = k;
         P.s0 = sa;
-        P.n_pix = std::min(pix_per_chunk, px1 - p0);
+        P.n_pix = std::min(pix_per_chunk, it.px1 - p0);
         P.n_paths = P.n_pix * k;
         P.sub_cap = f64 ? B64.sub_cap : B32.sub_cap;
         P.shadow_counts = kMaxBounce * kSubQueues;
         P.stat_slots = kStatSlots;
         P.counts = f64 ? B64.counts : B32.counts;
-        P.stats = ctx->stats;
+        P.stats = it.stats;
         const hipStream_t st = lane2 ? ctx->stream2 : ctx->stream;
         const hipEvent_t wait = used[lane ^ 1] ? ctx->acc_done[lane ^ 1] : nullptr;
         const hipEvent_t rec = (two || cancel) ? ctx->acc_done[lane] : nullptr;
         if (f64)
-          run_chunk(scene->dev, B64, P, scene->stack, st, side_of(ctx, lane2), hooks_of(ctx), wait, rec);
+          run_chunk(it.scene->dev, B64, P, it.scene->stack, st, side_of(ctx, lane2), hooks_of(ctx), wait, rec);
         else
-          run_chunk(scene->dev, B32, P, scene->stack, st, side_of(ctx, lane2), hooks_of(ctx), wait, rec);
+          run_chunk(it.scene->dev, B32, P, it.scene->stack, st, side_of(ctx, lane2), hooks_of(ctx), wait, rec);
         HIP_TRY(ctx, hipGetLastError());
         used[lane] = true;
-        if (two) lane ^= 1;
+        if (alt) lane ^= 1;
       }
       sa += k;
     }
@@ -960,6 +985,22 @@ rtg_status render_ranges(rtg_context* ctx, const rtg_scene* scene, const rtg_ren
     HIP_TRY(ctx, hipEventRecord(ctx->join, ctx->stream2));
     HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->join, 0));
   }
+  if (done_out) *done_out = done;
+  return cancelled ? RTG_ERR_CANCELLED : RTG_OK;
+}
+
+// Passes [bounds[i], bounds[i + 1]) of pixels [px0, px1), i < n_pass, of one
+// render (run_items: passes side by side, bitwise the passes one by one).
+rtg_status render_ranges(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* cfg,
+                         const uint32_t* bounds, uint32_t n_pass, uint32_t px0, uint32_t px1, rtg_accum* acc,
+                         const volatile int32_t* cancel, uint32_t* passes_done, rtg_stats* stats, const char* fn) {
+  std::vector<PassItem> items(n_pass);
+  for (uint32_t i = 0; i < n_pass; ++i) items[i] = PassItem{scene, cfg, acc, bounds[i], bounds[i + 1], px0, px1, ctx->stats};
+  HIP_TRY(ctx, hipMemsetAsync(ctx->stats, 0, sizeof(unsigned long long) * kStatSlots * kStatStride, ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  uint32_t done = 0;
+  const rtg_status rs = run_items(ctx, items.data(), n_pass, cancel, &done);
+  if (rs != RTG_OK && rs != RTG_ERR_CANCELLED) return rs;
   if (rtg_status st = stage_stats(ctx)) return st;
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
@@ -967,7 +1008,7 @@ rtg_status render_ranges(rtg_context* ctx, const rtg_scene* scene, const rtg_ren
   hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
   read_stats(ctx, stats, ms);
   if (passes_done) *passes_done = done;
-  if (cancelled) return fail(ctx, RTG_ERR_CANCELLED, std::string(fn) + ": cancelled");
+  if (rs == RTG_ERR_CANCELLED) return fail(ctx, RTG_ERR_CANCELLED, std::string(fn) + ": cancelled");
   return RTG_OK;
 }
 
@@ -1172,21 +1213,54 @@ rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jo
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<rtg_stats> acc_stats(n_jobs, rtg_stats{});
   // Jobs outside the batch kernels' scope, and jobs large enough to fill
-  // the GPU alone, through the single-render path.
+  // the GPU alone, through the single-render kernels: one after another with
+  // no host synchronisation (run_items, their chunks alternating between the
+  // lanes), each with its own statistics slots. Jobs write different
+  // accumulation buffers or disjoint pixels, so their order does not matter.
   uint64_t batch_paths = 0;
   std::vector<uint8_t> in_batch(n_jobs, 0);
+  std::vector<uint32_t> solo;
   for (uint32_t i = 0; i < n_jobs; ++i) {
     const uint64_t paths = (uint64_t)(jobs[i].s1 - jobs[i].s0) * (p1[i] - p0[i]);
     if (batchable(jobs[i]) && paths < kBatchSoloPaths) {
       in_batch[i] = 1;
       batch_paths += paths;
-      continue;
+    } else {
+      solo.push_back(i);
     }
-    rtg_stats st{};
-    if (rtg_status e = render_range(ctx, jobs[i].scene, &jobs[i].settings, jobs[i].s0, jobs[i].s1, p0[i], p1[i],
-                                    jobs[i].acc, nullptr, &st, fn.c_str()))
-      return e;
-    acc_stats[i] = st;
+  }
+  if (!solo.empty()) {
+    const uint32_t ns = (uint32_t)solo.size();
+    const size_t region = (size_t)kStatSlots * kStatStride;
+    if (ctx->job_stats_cap < ns) {
+      if (ctx->job_stats) HIP_TRY(ctx, hipFree(ctx->job_stats));
+      if (ctx->job_stats_host) HIP_TRY(ctx, hipHostFree(ctx->job_stats_host));
+      ctx->job_stats = nullptr;
+      ctx->job_stats_host = nullptr;
+      ctx->job_stats_cap = 0;
+      HIP_TRY(ctx, hipMalloc(&ctx->job_stats, sizeof(unsigned long long) * region * ns));
+      HIP_TRY(ctx, hipHostMalloc(&ctx->job_stats_host, sizeof(unsigned long long) * region * ns));
+      ctx->job_stats_cap = ns;
+    }
+    std::vector<PassItem> items(ns);
+    for (uint32_t k = 0; k < ns; ++k) {
+      const rtg_job& J = jobs[solo[k]];
+      items[k] = PassItem{J.scene, &J.settings, J.acc, J.s0, J.s1, p0[solo[k]], p1[solo[k]], ctx->job_stats + k * region};
+    }
+    HIP_TRY(ctx, hipMemsetAsync(ctx->job_stats, 0, sizeof(unsigned long long) * region * ns, ctx->stream));
+    if (rtg_status st = run_items(ctx, items.data(), ns, nullptr, nullptr)) return st;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->job_stats_host, ctx->job_stats, sizeof(unsigned long long) * region * ns,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned long long tot[kStCount] = {0};
+    for (uint32_t k = 0; k < ns; ++k) {
+      unsigned long long h[kStCount] = {0};
+      for (int sl = 0; sl < kStatSlots; ++sl)
+        for (int c = 0; c < kStCount; ++c) h[c] += ctx->job_stats_host[k * region + (size_t)sl * kStatStride + c];
+      add_counters(&acc_stats[solo[k]], h);
+      for (int c = 0; c < kStCount; ++c) tot[c] += h[c];
+    }
+    profile_resolve(ctx, tot);
   }
   if (batch_paths) {
     const uint32_t cap = capacity_for(batch_paths);
