@@ -92,37 +92,46 @@ __device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
 // Compacted appends of the whole block to two queues at once (the active
 // queue and the shadow queue): wave ballots, an LDS total per block, one
 // global atomic per queue per block on the block's own sub-queue counter.
-// Must be called by every thread of the block. Returns -1 or the slot.
+// The block's shadow entries are grouped by `cls` (the light they go to:
+// class 0 first), so the waves that trace them next hold rays to one light
+// from nearby points rather than a mix of directions. Must be called by
+// every thread of the block. Returns -1 or the slot.
 struct AppendSlots {
   int64_t a, b;
 };
 __device__ __forceinline__ AppendSlots block_append2(uint32_t* counts_a, uint32_t* counts_b, uint32_t sub_cap,
-                                                     bool pa, bool pb, uint32_t lb) {
-  __shared__ uint32_t s_wave[2][kBlock / 64];
+                                                     bool pa, bool pb, uint32_t cls, uint32_t lb) {
+  __shared__ uint32_t s_wave[3][kBlock / 64];
   __shared__ uint32_t s_base[2];
   const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const unsigned long long ma = __ballot(pa), mb = __ballot(pb);
+  const unsigned long long ma = __ballot(pa), mb0 = __ballot(pb && cls == 0), mb1 = __ballot(pb && cls != 0);
   if (lane == 0) {
     s_wave[0][wave] = (uint32_t)__popcll(ma);
-    s_wave[1][wave] = (uint32_t)__popcll(mb);
+    s_wave[1][wave] = (uint32_t)__popcll(mb0);
+    s_wave[2][wave] = (uint32_t)__popcll(mb1);
   }
   __syncthreads();
   const uint32_t sub = lb % kSubQueues;
   if (threadIdx.x < 2) {
     uint32_t tot = 0;
-    for (uint32_t w = 0; w < kBlock / 64; ++w) tot += s_wave[threadIdx.x][w];
+    for (uint32_t w = 0; w < kBlock / 64; ++w)
+      tot += threadIdx.x == 0 ? s_wave[0][w] : s_wave[1][w] + s_wave[2][w];
     uint32_t* c = (threadIdx.x == 0 ? counts_a : counts_b) + sub;
     s_base[threadIdx.x] = tot ? atomicAdd(c, tot) : 0u;
   }
   __syncthreads();
-  uint32_t oa = s_base[0], ob = s_base[1];
-  for (uint32_t w = 0; w < wave; ++w) {
-    oa += s_wave[0][w];
-    ob += s_wave[1][w];
+  uint32_t oa = s_base[0], ob0 = s_base[1], ob1 = s_base[1];
+  for (uint32_t w = 0; w < kBlock / 64; ++w) {
+    ob1 += s_wave[1][w];  // every class-0 entry of the block comes first
+    if (w < wave) {
+      oa += s_wave[0][w];
+      ob0 += s_wave[1][w];
+      ob1 += s_wave[2][w];
+    }
   }
   AppendSlots r;
   r.a = pa ? (int64_t)sub * sub_cap + oa + lanes_below(ma) : -1;
-  r.b = pb ? (int64_t)sub * sub_cap + ob + lanes_below(mb) : -1;
+  r.b = pb ? (int64_t)sub * sub_cap + (cls == 0 ? ob0 + lanes_below(mb0) : ob1 + lanes_below(mb1)) : -1;
   __syncthreads();  // s_wave / s_base are reused by the block's next job
   return r;
 }
@@ -381,7 +390,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
   const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
   bool alive = false, shadow = false;
-  uint32_t pid = 0;
+  uint32_t pid = 0, shadow_cls = 0;
   ShadowReq<T> req;
   if (i < qv.total) {
     pid = active_pid(B, P, pre, b, i);
@@ -409,6 +418,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
           const uint32_t li = rng.below(nl);
           if (ModeOps<T>::light(S, li, rec, mat, rng, req)) {
             shadow = true;
+            shadow_cls = li & 1u;
             req.f_rop = (beta * req.f_rop) * T(nl);
           }
         }
@@ -440,7 +450,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
   // Converged: compacted appends to the next active queue and the shadow queue.
   const AppendSlots a = block_append2(P.counts + (size_t)(b + 1) * kSubQueues,
                                       P.counts + P.shadow_counts + (size_t)b * kSubQueues, P.sub_cap, alive, shadow,
-                                      lb);
+                                      shadow_cls, lb);
   if (alive) B.q[(b + 1) & 1][P.q_base + a.a] = pid;
   if (shadow) {
     const int64_t si = P.q_base + a.b + (int64_t)(b & 1) * B.sq_stride;
