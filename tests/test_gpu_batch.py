@@ -113,7 +113,9 @@ def test_jobs_outside_the_batch_kernels(ctx):
     for (o, cfg, spp), (s, c), st in zip(sp, res, stats):
         s1, c1, st1 = solo(ctx, o, cfg, 0, spp)
         assert np.array_equal(c, c1) and np.array_equal(s, s1)
-        assert st.rays == st1.rays
+        # the single-render jobs run side by side, each with its own counters
+        for k in COUNTERS:
+            assert getattr(st, k) == getattr(st1, k), k
 
 
 def test_many_small_jobs_in_several_groups(ctx, specs, monkeypatch):
