@@ -176,6 +176,16 @@ rtg_status rtg_device_info(rtg_context* ctx, char* name, size_t name_len, int* m
  * (float bounds rounded outward) and the primitives into leaf order.
  * Replaces the scene argument of Renderer (rendering/renderer.hpp:52). */
 rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* desc, rtg_scene** out);
+/* The FP32 node records rtg_scene_create builds from `desc`, computed on
+ * the host (no device is touched): 16 floats per interior node of the
+ * reference BVH, in its DFS order — left child box lo.xyz, hi.xyz, right
+ * child box lo.xyz, hi.xyz (each rounded outward from the double box, so it
+ * contains it), then the left and right child refs and the split axis as
+ * u32 bit patterns, and a zero. Writes min(cap, *n) records to `out` (may be
+ * NULL) and the record count to *n. The layout replaces BvhNode
+ * (include/raytracer/geometry/bvh.hpp:20-26) on the device; this entry point
+ * exposes it to tools and tests. */
+rtg_status rtg_scene_node_records(const rtg_scene_desc* desc, float* out, uint64_t cap, uint64_t* n);
 rtg_status rtg_scene_release(rtg_scene* scene);
 /* A handle on `base`'s device arrays with another camera — the same scene
  * rendered at another image size needs the camera set up for that size

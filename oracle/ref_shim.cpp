@@ -316,6 +316,39 @@ void ref_render_rows(void* h, uint32_t s0, uint32_t s1, int y0, int y1, double* 
   }
 }
 
+// Same loop as ref_render_rows, also summing each sample's squared radiance
+// per channel (sumsq), so the fixtures can state each pixel's sample
+// variance; the reference Framebuffer keeps only sum and count
+// (framebuffer.hpp:48-49).
+void ref_render_moments(void* h, uint32_t s0, uint32_t s1, int y0, int y1, double* sum, double* sumsq, uint32_t* count,
+                        uint64_t* stats /* 5 */) {
+  const Scene& s = static_cast<RefScene*>(h)->scene;
+  const auto& cfg = s.settings;
+  PathStats st;
+  uint64_t samples = 0;
+  for (int y = y0; y < y1; ++y)
+    for (int x = 0; x < cfg.width; ++x) {
+      const auto pixel = static_cast<std::uint64_t>(y) * static_cast<std::uint64_t>(cfg.width) + static_cast<std::uint64_t>(x);
+      const std::size_t i = (std::size_t)pixel;
+      for (std::uint32_t smp = s0; smp < s1; ++smp) {
+        Rng rng = Rng::for_sample(cfg.seed, pixel, smp);
+        const Real jx = rng.uniform();
+        const Real jy = rng.uniform();
+        const Ray r = s.camera.generate(x, y, jx, jy, rng);
+        Color L = radiance(s, r, rng, st);
+        if (!L.is_finite() || L.x < 0 || L.y < 0 || L.z < 0) { st.invalid_samples++; L = Color{}; }
+        sum[3 * i] += L.x; sum[3 * i + 1] += L.y; sum[3 * i + 2] += L.z;
+        sumsq[3 * i] += L.x * L.x; sumsq[3 * i + 1] += L.y * L.y; sumsq[3 * i + 2] += L.z * L.z;
+        count[i]++;
+        ++samples;
+      }
+    }
+  if (stats) {
+    stats[0] += samples; stats[1] += st.rays; stats[2] += st.shadow_rays; stats[3] += st.rr_terminated;
+    stats[4] += st.invalid_samples;
+  }
+}
+
 // The reference Renderer end to end (thread pool, tiles, progressive
 // passes): the CPU baseline. Writes the per-pixel mean if `mean` is set.
 int ref_render_full(void* h, int threads, int progressive, int max_pass, double* mean, uint64_t* stats /* 5 */,

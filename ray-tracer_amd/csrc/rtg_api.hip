@@ -281,6 +281,60 @@ const LaunchHooks* hooks_of(rtg_context* ctx) {
   return &ctx->hooks;
 }
 
+// The FP32 two-child records of the reference BVH's interior nodes, in its
+// DFS order (node i's record is the i-th interior node's): both child boxes
+// rounded outward from the double boxes, so each float box contains its
+// double box, and the two child refs (leaf: first slot + count; interior:
+// record index) and the split axis. max_depth = the BVH's depth.
+bool build_node_records(const rtg_scene_desc* d, std::vector<float4>& node2, uint32_t& max_depth,
+                        std::string& err) {
+  std::vector<int32_t> n2_index(d->n_nodes, -1);
+  uint32_t n_interior = 0;
+  max_depth = 0;
+  std::vector<std::pair<uint32_t, uint32_t>> st;  // (node, depth)
+  if (d->n_nodes) st.push_back({0, 0});
+  while (!st.empty()) {
+    auto [i, dep] = st.back();
+    st.pop_back();
+    max_depth = std::max(max_depth, dep);
+    const rtg_bvh_node& n = d->nodes[i];
+    if (n.count == 0) {
+      if (n.offset >= d->n_nodes || i + 1 >= d->n_nodes) {
+        err = "scene: BVH child index out of range";
+        return false;
+      }
+      st.push_back({n.offset, dep + 1});
+      st.push_back({i + 1, dep + 1});
+    } else if (n.count > kMaxLeafCount) {
+      err = "scene: BVH leaf with more than 64 primitives";
+      return false;
+    }
+    if (dep > 63) {
+      err = "scene: BVH deeper than 63 levels";
+      return false;
+    }
+  }
+  for (uint32_t i = 0; i < d->n_nodes; ++i)
+    if (d->nodes[i].count == 0) n2_index[i] = (int32_t)n_interior++;
+  auto child_ref = [&](uint32_t c) -> uint32_t {
+    const rtg_bvh_node& n = d->nodes[c];
+    return n.count > 0 ? leaf_ref(n.offset, n.count) : (uint32_t)n2_index[c];
+  };
+  node2.assign((size_t)n_interior * kNode2Stride, f4(0, 0, 0, 0));
+  for (uint32_t i = 0; i < d->n_nodes; ++i) {
+    const rtg_bvh_node& n = d->nodes[i];
+    if (n.count > 0) continue;
+    const rtg_bvh_node& L = d->nodes[i + 1];
+    const rtg_bvh_node& R = d->nodes[n.offset];
+    float4* o = &node2[(size_t)n2_index[i] * kNode2Stride];
+    o[0] = f4(round_down(L.lo.x), round_down(L.lo.y), round_down(L.lo.z), round_up(L.hi.x));
+    o[1] = f4(round_up(L.hi.y), round_up(L.hi.z), round_down(R.lo.x), round_down(R.lo.y));
+    o[2] = f4(round_down(R.lo.z), round_up(R.hi.x), round_up(R.hi.y), round_up(R.hi.z));
+    o[3] = f4(bits_f(child_ref(i + 1)), bits_f(child_ref(n.offset)), bits_f(n.axis), 0.0f);
+  }
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -405,6 +459,18 @@ rtg_status rtg_synchronize(rtg_context* ctx) {
   return RTG_OK;
 }
 
+rtg_status rtg_scene_node_records(const rtg_scene_desc* desc, float* out, uint64_t cap, uint64_t* n) {
+  if (!desc || !n) return fail(nullptr, RTG_ERR_INVALID_ARG, "rtg_scene_node_records: NULL argument");
+  if (desc->n_nodes && !desc->nodes) return fail(nullptr, RTG_ERR_INVALID_ARG, "scene: null array with non-zero count");
+  std::vector<float4> node2;
+  uint32_t depth = 0;
+  std::string err;
+  if (!build_node_records(desc, node2, depth, err)) return fail(nullptr, RTG_ERR_UNSUPPORTED, err);
+  *n = node2.size() / kNode2Stride;
+  if (out) std::memcpy(out, node2.data(), std::min<uint64_t>(cap, *n) * kNode2Stride * sizeof(float4));
+  return RTG_OK;
+}
+
 // ---------------------------------------------------------------------------
 rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* d, rtg_scene** out) {
   if (!ctx || !d || !out) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_scene_create: NULL argument");
@@ -423,46 +489,14 @@ rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* d, rtg_scene
   const uint32_t np = d->n_prims;
 
   // --- BVH depth and the FP32 two-child node layout ---
-  std::vector<int32_t> n2_index(d->n_nodes, -1);
-  uint32_t n_interior = 0, max_depth = 0;
+  std::vector<float4> node2;
+  uint32_t max_depth = 0;
   {
-    std::vector<std::pair<uint32_t, uint32_t>> st;  // (node, depth)
-    if (d->n_nodes) st.push_back({0, 0});
-    while (!st.empty()) {
-      auto [i, dep] = st.back();
-      st.pop_back();
-      max_depth = std::max(max_depth, dep);
-      const rtg_bvh_node& n = d->nodes[i];
-      if (n.count == 0) {
-        st.push_back({n.offset, dep + 1});
-        st.push_back({i + 1, dep + 1});
-      } else if (n.count > kMaxLeafCount) {
-        return cleanup(fail(ctx, RTG_ERR_UNSUPPORTED, "scene: BVH leaf with more than 64 primitives"));
-      }
-    }
-    for (uint32_t i = 0; i < d->n_nodes; ++i)
-      if (d->nodes[i].count == 0) n2_index[i] = (int32_t)n_interior++;
+    std::string err;
+    if (!build_node_records(d, node2, max_depth, err)) return cleanup(fail(ctx, RTG_ERR_UNSUPPORTED, err));
   }
   if (np > kMaxSlots) return cleanup(fail(ctx, RTG_ERR_UNSUPPORTED, "scene: more than 2^25 primitives"));
   s->stack = max_depth + 1;
-  if (s->stack > 64) return cleanup(fail(ctx, RTG_ERR_UNSUPPORTED, "scene: BVH deeper than 63 levels"));
-
-  auto child_ref = [&](uint32_t c) -> uint32_t {
-    const rtg_bvh_node& n = d->nodes[c];
-    return n.count > 0 ? leaf_ref(n.offset, n.count) : (uint32_t)n2_index[c];
-  };
-  std::vector<float4> node2((size_t)n_interior * kNode2Stride);
-  for (uint32_t i = 0; i < d->n_nodes; ++i) {
-    const rtg_bvh_node& n = d->nodes[i];
-    if (n.count > 0) continue;
-    const rtg_bvh_node& L = d->nodes[i + 1];
-    const rtg_bvh_node& R = d->nodes[n.offset];
-    float4* o = &node2[(size_t)n2_index[i] * kNode2Stride];
-    o[0] = f4(round_down(L.lo.x), round_down(L.lo.y), round_down(L.lo.z), round_up(L.hi.x));
-    o[1] = f4(round_up(L.hi.y), round_up(L.hi.z), round_down(R.lo.x), round_down(R.lo.y));
-    o[2] = f4(round_down(R.lo.z), round_up(R.hi.x), round_up(R.hi.y), round_up(R.hi.z));
-    o[3] = f4(bits_f(child_ref(i + 1)), bits_f(child_ref(n.offset)), bits_f(n.axis), 0.0f);
-  }
 
   // --- slots: leaf order first, then primitives outside the BVH ---
   std::vector<uint32_t> prim_slot(np, 0xFFFFFFFFu);

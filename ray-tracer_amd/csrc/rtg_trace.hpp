@@ -282,9 +282,21 @@ __device__ inline bool any64(const DevScene& S, bool use_bvh, V3<double> o, V3<d
 
 // =========================== FP32 (fast) ================================
 
+// Slab distances are one fma each: the distance to the plane x = b is
+// fma(b, 1/d, -q) with q = o * (1/d). That rounds with a relative error
+// below 1.5 * 2^-23 of the distance (1/d and the fma) plus an absolute error
+// below |q| * 2^-24 (the rounding of q), which does not shrink with the
+// distance: at the Cornell box's 555-unit scale it is ~1e-4 for a box a few
+// units away. So a slab's lower plane is measured from q moved by a margin
+// m = |q| * 2^-21 one way and its upper plane from q moved the other way (lq
+// for a box's lo planes, hq for its hi planes; the sign of 1/d decides which
+// plane is entered first): every entry moves down and every exit up by at
+// least 4x the absolute error, at no cost per box. The relative part is
+// covered by widening the exit distance (box32).
 struct Ray32 {
   float ox, oy, oz, dx, dy, dz, ix, iy, iz;
-  float qx, qy, qz;  // o / d: slab distances are one fma each
+  float lqx, lqy, lqz;  // offsets of the lo planes
+  float hqx, hqy, hqz;  // offsets of the hi planes
 };
 
 __device__ __forceinline__ float safe_inv(float d) {
@@ -299,19 +311,24 @@ __device__ __forceinline__ Ray32 make_ray32(float ox, float oy, float oz, float 
   r.ox = ox; r.oy = oy; r.oz = oz;
   r.dx = dx; r.dy = dy; r.dz = dz;
   r.ix = safe_inv(dx); r.iy = safe_inv(dy); r.iz = safe_inv(dz);
-  r.qx = ox * r.ix; r.qy = oy * r.iy; r.qz = oz * r.iz;
+  const float qx = ox * r.ix, qy = oy * r.iy, qz = oz * r.iz;
+  constexpr float k = 0x1p-21f;
+  const float mx = r_copysign(r_fabs(qx) * k, r.ix), my = r_copysign(r_fabs(qy) * k, r.iy),
+              mz = r_copysign(r_fabs(qz) * k, r.iz);
+  r.lqx = qx + mx; r.lqy = qy + my; r.lqz = qz + mz;
+  r.hqx = qx - mx; r.hqy = qy - my; r.hqz = qz - mz;
   return r;
 }
 
 // Slab test of one float box; returns the entry distance in tn. Plane
-// distances are fma(bound, 1/d, -o/d); the far distance is widened by
-// 5e-7 relative so that the rounding of o/d never culls a box the ray
-// touches.
+// distances are fma(bound, 1/d, -offset) with the ray's margins folded into
+// the offsets (Ray32); the far distance is also widened by 5e-7 relative, so
+// rounding never culls a box the ray touches.
 __device__ __forceinline__ bool box32(const Ray32& r, float lx, float ly, float lz, float hx, float hy, float hz,
                                       float tmin, float tmax, float& tn) {
-  const float t0x = __builtin_fmaf(lx, r.ix, -r.qx), t1x = __builtin_fmaf(hx, r.ix, -r.qx);
-  const float t0y = __builtin_fmaf(ly, r.iy, -r.qy), t1y = __builtin_fmaf(hy, r.iy, -r.qy);
-  const float t0z = __builtin_fmaf(lz, r.iz, -r.qz), t1z = __builtin_fmaf(hz, r.iz, -r.qz);
+  const float t0x = __builtin_fmaf(lx, r.ix, -r.lqx), t1x = __builtin_fmaf(hx, r.ix, -r.hqx);
+  const float t0y = __builtin_fmaf(ly, r.iy, -r.lqy), t1y = __builtin_fmaf(hy, r.iy, -r.hqy);
+  const float t0z = __builtin_fmaf(lz, r.iz, -r.lqz), t1z = __builtin_fmaf(hz, r.iz, -r.hqz);
   const float nx = r_fmin(t0x, t1x), fx = r_fmax(t0x, t1x);
   const float ny = r_fmin(t0y, t1y), fy = r_fmax(t0y, t1y);
   const float nz = r_fmin(t0z, t1z), fz = r_fmax(t0z, t1z);

@@ -135,6 +135,7 @@ def load_rtg():
         "rtg_abi_version": (C.c_int, []),
         "rtg_device_info": (C.c_int32, [_VP, C.c_char_p, C.c_size_t, P(C.c_int), P(C.c_uint64)]),
         "rtg_scene_create": (C.c_int32, [_VP, P(SceneDesc), P(_VP)]),
+        "rtg_scene_node_records": (C.c_int32, [P(SceneDesc), _VP, C.c_uint64, P(C.c_uint64)]),
         "rtg_scene_release": (C.c_int32, [_VP]),
         "rtg_scene_with_camera": (C.c_int32, [_VP, _VP, P(Camera), P(_VP)]),
         "rtg_scene_bytes": (C.c_uint64, [_VP]),
@@ -166,6 +167,7 @@ def load_rtg():
 def rtg_symbols():
     """Names declared in include/rtg.h (checked by the CPU tests)."""
     return ["rtg_open", "rtg_close", "rtg_last_error", "rtg_abi_version", "rtg_device_info", "rtg_scene_create",
+            "rtg_scene_node_records",
             "rtg_scene_release", "rtg_scene_with_camera", "rtg_scene_bytes", "rtg_accum_create", "rtg_accum_release", "rtg_accum_clear",
             "rtg_accum_read", "rtg_accum_min_samples", "rtg_render_pass", "rtg_render_passes", "rtg_render_batch",
             "rtg_resolve",
@@ -176,6 +178,20 @@ def rtg_symbols():
 
 def _ptr(a):
     return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def node_records(desc: "SceneDesc"):
+    """The FP32 node records rtg_scene_create builds from `desc` (host only):
+    an (n, 16) float32 array (rtg_scene_node_records)."""
+    lib = load_rtg()
+    n = C.c_uint64(0)
+    st = lib.rtg_scene_node_records(C.byref(desc), None, 0, C.byref(n))
+    if st != 0:
+        raise RtgError(f"rtg status {st}: {lib.rtg_last_error(None).decode()}")
+    out = np.zeros((n.value, 16), np.float32)
+    if n.value:
+        lib.rtg_scene_node_records(C.byref(desc), _ptr(out), n.value, C.byref(n))
+    return out
 
 
 class Context:

@@ -126,6 +126,8 @@ def ref():
             "ref_camera_rays": (None, [VP, VP, C.c_uint64, C.c_uint64, VP]),
             "ref_trace_batch": (None, [VP, C.c_int, C.c_int, VP, C.c_uint64, VP, P(C.c_uint64), P(C.c_uint64)]),
             "ref_render_rows": (None, [VP, C.c_uint32, C.c_uint32, C.c_int, C.c_int, VP, VP, P(C.c_uint64)]),
+            "ref_render_moments": (None, [VP, C.c_uint32, C.c_uint32, C.c_int, C.c_int, VP, VP, VP,
+                                          P(C.c_uint64)]),
             "ref_render_full": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, VP, P(C.c_uint64), P(C.c_double)]),
             "ref_rng_stream": (None, [C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_uint32), C.c_int]),
             "ref_bvh_build": (C.c_int, [P(C.c_double), C.c_uint32, C.c_int, P(rtg.BvhNode), P(C.c_uint32),
@@ -370,6 +372,31 @@ class RefScene:
         [t.start() for t in ts]
         [t.join() for t in ts]
         return s, c, [sum(st[i] for st in stats) for i in range(5)]
+
+    def render_moments(self, s0, s1, w, h, threads=8):
+        """Per-pixel sum, sum of squares and count of the reference's samples
+        [s0, s1) (ref_render_moments); rows are handed out one at a time."""
+        s = np.zeros((h, w, 3))
+        q = np.zeros((h, w, 3))
+        c = np.zeros((h, w), np.uint32)
+        stats = [(C.c_uint64 * 5)() for _ in range(threads)]
+        nxt = [0]
+        lock = threading.Lock()
+
+        def work(k):
+            while True:
+                with lock:
+                    y = nxt[0]
+                    nxt[0] += 1
+                if y >= h:
+                    return
+                ref().ref_render_moments(self.h, s0, s1, y, y + 1, C.c_void_p(s.ctypes.data),
+                                         C.c_void_p(q.ctypes.data), C.c_void_p(c.ctypes.data), stats[k])
+
+        ts = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+        [t.start() for t in ts]
+        [t.join() for t in ts]
+        return s, q, c, [sum(st[i] for st in stats) for i in range(5)]
 
 
 def ray_set(desc, n, seed=7):

@@ -4,8 +4,11 @@ twice the scene box (tests/orc.py ray_set), closest hit on the GPU.
 FP64 mode must reproduce the reference's Scene::intersect for every ray —
 the SHA-256 of all 16,777,216 hit ids and of all t values, the hit count and
 the traversal counters were computed by the reference build
-(tests/golden/make_golden.py). Brute force (--no-bvh) must give the same ids
-as the BVH. FP32 mismatches are counted and bounded."""
+(tests/golden/make_golden.py). Brute force (--no-bvh, scene.cpp:200-215)
+must give the reference's hits on the first 65,536 rays (fixture) and, over
+all 2^24 rays, the same t as the BVH bit for bit and the same ids except at
+exact ties, where the reference's own brute force decides. FP32 mismatches
+are counted and bounded."""
 import json
 import os
 
@@ -53,13 +56,48 @@ def test_brute_force_matches_bvh(ctx, torus):
     assert np.array_equal(hb["t"].view(np.uint64), want["t"].view(np.uint64))
 
 
+def test_brute_force_all_rays(ctx, torus):
+    """Scene::intersect_brute (scene.cpp:200-215) over all 2^24 rays: every
+    t bit-identical to the BVH's (whose ids and t hash to the reference's),
+    the same hits, and the same ids but where two primitives tie at exactly
+    the same t — there the lowest index wins in brute force, and the
+    reference's own brute force (oracle/_ref, or the oracle restating it) must
+    pick what the device picked."""
+    hs, dev, rays = torus
+    hv, _ = ctx.trace(dev, rays, mode=0, use_bvh=1, precision=rtg.PRECISION_F64)
+    assert orc.ids_digest(hv) == (META["ids_sha256"], META["t_sha256"])
+    hb = np.zeros(len(rays), rtg.HIT_DTYPE)
+    ms = 0.0
+    step = 1 << 21
+    for k in range(0, len(rays), step):
+        print(f"brute force rays [{k}, {k + step})", flush=True)
+        h, st = ctx.trace(dev, rays[k:k + step], mode=0, use_bvh=0, precision=rtg.PRECISION_F64)
+        assert st.prim_tests == len(h) * 172800
+        hb[k:k + step] = h
+        ms += st.elapsed_ms
+    assert np.array_equal(hb["flags"], hv["flags"])
+    assert np.array_equal(hb["t"].view(np.uint64), hv["t"].view(np.uint64))
+    diff = np.flatnonzero(hb["prim"] != hv["prim"])
+    print(f"brute force fp64: 2^24 rays x 172800 in {ms:.0f} ms device; {len(diff)} ids differ from the BVH's "
+          f"at equal t", flush=True)
+    if len(diff):
+        judge = orc.RefScene(scenes.torus_only()) if orc.have_ref() else orc.OracleScene(scenes.torus_only())
+        want, _, _ = judge.trace(rays[diff], mode=0, use_bvh=0)
+        assert np.array_equal(want["prim"], hb["prim"][diff])
+        assert np.array_equal(want["t"].view(np.uint64), hb["t"][diff].view(np.uint64))
+    else:
+        assert orc.ids_digest(hb) == (META["ids_sha256"], META["t_sha256"])
+
+
 def test_f32_mismatches_are_rare(ctx, torus):
     hs, dev, rays = torus
     h64, _ = ctx.trace(dev, rays, mode=0, precision=rtg.PRECISION_F64)
     h32, st = ctx.trace(dev, rays, mode=0, precision=rtg.PRECISION_F32)
     mism = int((h32["prim"] != h64["prim"]).sum())
     print(f"2^24 rays fp32: {st.elapsed_ms:.1f} ms device, id mismatches vs fp64: {mism}", flush=True)
-    assert mism <= len(rays) * 1e-4
+    # 23-25 are seen: grazing rays whose FP32 hit lands on a neighbouring
+    # triangle of the shared edge
+    assert mism <= 64
     both = (h32["prim"] == h64["prim"]) & ((h64["flags"] & 1) != 0)
     # FP32 hit points carry float rounding at the scene's scale (extent ~1.35,
     # ulp ~1.2e-7): the error of t projected on the triangle's normal, i.e. the
@@ -78,6 +116,20 @@ def test_f32_mismatches_are_rare(ctx, torus):
           f"{(dt <= 1e-5).mean():.6f} of hits within 1e-5", flush=True)
     assert off.max() <= 4e-6
     assert (dt <= 1e-5).mean() >= 0.9999
+
+
+def test_f32_walk_culls_no_box(ctx, torus):
+    """The FP32 walk's closest t equals FP32 brute force's (the same slot
+    test over all 172,800 triangles) bit for bit on 4 Mi rays: slab rounding
+    culls no box that holds the closest hit."""
+    hs, dev, rays = torus
+    sub = rays[:1 << 22]
+    hb, _ = ctx.trace(dev, sub, mode=0, use_bvh=1, precision=rtg.PRECISION_F32)
+    hr, st = ctx.trace(dev, sub, mode=0, use_bvh=0, precision=rtg.PRECISION_F32)
+    print(f"fp32 brute force: {len(sub)} rays in {st.elapsed_ms:.0f} ms", flush=True)
+    assert np.array_equal(hb["flags"], hr["flags"])
+    assert np.array_equal(hb["t"].view(np.uint64), hr["t"].view(np.uint64))
+    assert np.array_equal(hb["prim"], hr["prim"])
 
 
 def _unpermute(hits, perm):

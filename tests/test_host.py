@@ -117,3 +117,43 @@ def test_rt_cli_cpu_paths():
     assert r.returncode == 1 and b"unknown option" in r.stderr
     r = subprocess.run([RT], capture_output=True, timeout=60)
     assert r.returncode == 1
+
+
+NODE_SCENES = [("cornell_c1.json", None), ("torus_c2.json", None), ("sphere_field_c3.json", None), (None, 1),
+               (None, 2), (None, 3)]
+
+
+@pytest.mark.parametrize("name,seed", NODE_SCENES)
+def test_fp32_node_boxes_contain_the_double_boxes(name, seed):
+    """Every FP32 child box of the device's node records contains the double
+    box of the reference BVH node it stands for (include/raytracer/geometry/
+    bvh.hpp:20-26), child refs point at the right records or leaf ranges, and
+    the split axis is the node's."""
+    j = scenes.load(name) if name else scenes.random_scene_json(300, 600, seed, seed == 2)
+    hs = rtg.HostScene(json.dumps(j))
+    d = hs.desc
+    nodes = np.frombuffer((rtg.BvhNode * d.n_nodes).from_address(C.addressof(d.nodes.contents)),
+                          dtype=np.dtype([("lo", "<f8", 3), ("hi", "<f8", 3), ("offset", "<u4"), ("count", "<u2"),
+                                          ("axis", "u1"), ("pad", "u1")]))
+    rec = rtg.node_records(d)
+    interior = np.flatnonzero(nodes["count"] == 0)
+    assert len(rec) == len(interior) > 0
+    record_of = np.full(len(nodes), -1, np.int64)
+    record_of[interior] = np.arange(len(interior))
+    left, right = interior + 1, nodes["offset"][interior].astype(np.int64)
+    f = rec[:, :12].astype(np.float64)
+    for child, lo, hi in ((left, f[:, 0:3], f[:, 3:6]), (right, f[:, 6:9], f[:, 9:12])):
+        assert (lo <= nodes["lo"][child]).all() and (hi >= nodes["hi"][child]).all()
+        # outward rounding is tight: one float step at most
+        assert (np.nextafter(lo.astype(np.float32), np.float32(np.inf)).astype(np.float64) > nodes["lo"][child]).all()
+        assert (np.nextafter(hi.astype(np.float32), np.float32(-np.inf)).astype(np.float64) < nodes["hi"][child]).all()
+    refs = rec[:, 12:15].view(np.uint32)
+    assert np.array_equal(refs[:, 2], nodes["axis"][interior])
+    for col, child in ((0, left), (1, right)):
+        r = refs[:, col]
+        leaf = nodes["count"][child] > 0
+        assert np.array_equal((r >> 31) == 1, leaf)
+        assert np.array_equal(r[~leaf], record_of[child[~leaf]])
+        assert np.array_equal(r[leaf] & 0x1FFFFFF, nodes["offset"][child[leaf]])
+        assert np.array_equal(((r[leaf] >> 25) & 63) + 1, nodes["count"][child[leaf]])
+    hs.close()
