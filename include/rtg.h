@@ -229,6 +229,28 @@ rtg_status rtg_render_passes(rtg_context* ctx, const rtg_scene* scene, const rtg
                              const uint32_t* bounds, uint32_t n_passes, rtg_accum* acc,
                              const volatile int32_t* cancel, uint32_t* passes_done, rtg_stats* stats);
 
+/* rtg_render_pass without device allocation. This entry point has no
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The path state and statistics slots live
+ * in a fixed device array of RTG_STATIC_SCRATCH_BYTES inside librtg.so, one
+ * copy per device, held until the process ends; the call makes no
+ * hipMalloc / hipFree and clears what must start at zero with a kernel.
+ * A pass of any size is accepted: it is rendered on one lane in chunks of at
+ * most 256 Ki paths (the path state that fits the array, in either
+ * precision), each pixel's samples added in order, so the accumulation
+ * buffer and the counters are bitwise those of rtg_render_pass. Every call on one
+ * device shares that device's array: calls that overlap on one device (from
+ * contexts on different host threads) race and must not be issued.
+ * With stats == NULL and cancel == NULL the call returns once its work is
+ * enqueued on the context's stream (rtg_synchronize or a later call on the
+ * context orders after it); otherwise it waits for the pass, as
+ * rtg_render_pass does. Arguments and errors as rtg_render_pass. */
+#define RTG_STATIC_SCRATCH_BYTES (64ull << 20)
+rtg_status rtg_render_pass_static(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* settings,
+                                  uint32_t s0, uint32_t s1, rtg_accum* acc, const volatile int32_t* cancel,
+                                  rtg_stats* stats);
+
 /* Tone map + sRGB encode on the device (rendering/tonemap.cpp:14-53):
  * writes width*height*3 bytes. tone_map: 0 clamp, 1 reinhard, 2 aces. */
 rtg_status rtg_resolve(rtg_context* ctx, const rtg_accum* acc, int32_t tone_map, double exposure, uint8_t* rgb);

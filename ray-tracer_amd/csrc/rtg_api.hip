@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -16,8 +17,42 @@
 
 using namespace rtg;
 
+// Device scratch of rtg_render_pass_static, the entry point that allocates
+// nothing: the statistics slots and one chunk's path state live in this
+// module-scope array. Every device the library runs on holds its own copy
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (static_scratch_of), never by
+// the array's name, which on the host is not a device address.
+constexpr uint64_t kStaticScratchBytes = RTG_STATIC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
 namespace {
 thread_local std::string g_error;
+
+// Address of `scratch` on each device, looked up once per device.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch[kMaxDevices] = {};
+
+// With `device` current: its copy of the scratch array.
+hipError_t static_scratch_of(int device, char** out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = static_cast<char*>(p);
+  }
+  *out = g_scratch[device];
+  return hipSuccess;
+}
+
+// Zero fill (a kernel, so a caller may capture it in a graph).
+__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
 
 // Path-state slots of a context.
 constexpr int kBatchBuffers = 2;
@@ -88,6 +123,7 @@ struct rtg_context {
   std::vector<int> event_kind;  // kind of each begin event
   rtg_profile profile{};
   LaunchHooks hooks{};
+  char* scratch_dev = nullptr;  // this device's copy of `scratch` (rtg_render_pass_static)
 };
 
 void drop_batch_graphs(rtg_context* ctx) {
@@ -393,6 +429,7 @@ rtg_status rtg_open(int device, rtg_context** out) {
   if (e == hipSuccess) e = hipMalloc(&c->stats, sizeof(unsigned long long) * kStatSlots * kStatStride);
   if (e == hipSuccess) e = hipHostMalloc(&c->stats_host, sizeof(unsigned long long) * kStatSlots * kStatStride);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+  if (e == hipSuccess) e = static_scratch_of(device, &c->scratch_dev);
   if (e != hipSuccess) {
     const std::string m = std::string("rtg_open: ") + hipGetErrorString(e);
     delete c;
@@ -742,12 +779,65 @@ rtg_status rtg_resolve(rtg_context* ctx, const rtg_accum* a, int32_t tone_map, d
 // ---------------------------------------------------------------------------
 namespace {
 
-// Path state for `cap` paths. Every array has c entries: c = room for all
-// sub-queues of one chunk, or (batches) of every job of a batch.
+// Path state for `cap` paths laid out from `base` (the path arrays,
+// sub-queues and queue counters, and for batches the statistics slots).
+// Every array has c entries: c = room for all sub-queues of one chunk, or
+// (batches) of every job of a batch. Returns the bytes the layout takes;
+// with base == nullptr only the size is computed and B is left alone.
+template <class T>
+uint64_t lay_out_paths(char* base, uint32_t cap, bool batch, unsigned long long* stats, PathBuffers<T>* B) {
+  const uint32_t sub_cap = sub_queue_cap(cap);
+  const uint64_t c = batch ? (uint64_t)cap + kBatchQueueSlack : (uint64_t)sub_cap * kSubQueues;  // >= cap
+  auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
+  // 6 ray + 3 beta + 3 L + 3 hit + 10 shadow arrays of T; 2 RNG u64; 5 u32
+  // (2 queues, hprim, hplane, sq_pid); spec; counts. FP32: a second shadow
+  // queue (10 T + sq_pid) and 3 shadow-radiance arrays.
+  constexpr bool f32 = sizeof(T) == 4;
+  const uint64_t sq = f32 ? 2 * c : c;  // shadow-queue entries
+  const uint64_t n_counts = batch ? (uint64_t)kMaxBatchJobs * kBatchCounts : 2ull * kMaxBounce * kSubQueues;
+  const uint64_t n_stats = batch ? (uint64_t)kMaxBatchJobs * kBatchStatSlots * kStatStride : 0;
+  const uint64_t bytes = al(c * sizeof(T)) * (15 + (f32 ? 3 : 0)) + al(sq * sizeof(T)) * 10 + al(c * 8) * 2 +
+                         al(c * 4) * 4 + al(sq * 4) + al(c) + al(sizeof(uint32_t) * n_counts) + al(8 * n_stats);
+  if (!base) return bytes;
+  char* p = base;
+  auto take = [&](uint64_t b) {
+    char* r = p;
+    p += al(b);
+    return r;
+  };
+  T** tarr[] = {&B->ox, &B->oy, &B->oz, &B->dx, &B->dy, &B->dz, &B->bx, &B->by, &B->bz, &B->lx, &B->ly, &B->lz,
+                &B->ht, &B->hb1, &B->hb2};
+  for (T** a : tarr) *a = reinterpret_cast<T*>(take(c * sizeof(T)));
+  T** sarr[] = {&B->sox, &B->soy, &B->soz, &B->sdx, &B->sdy, &B->sdz, &B->stm, &B->scx, &B->scy, &B->scz};
+  for (T** a : sarr) *a = reinterpret_cast<T*>(take(sq * sizeof(T)));
+  B->slx = B->sly = B->slz = nullptr;
+  if (f32) {
+    B->slx = reinterpret_cast<T*>(take(c * sizeof(T)));
+    B->sly = reinterpret_cast<T*>(take(c * sizeof(T)));
+    B->slz = reinterpret_cast<T*>(take(c * sizeof(T)));
+  }
+  B->sq_stride = f32 ? (uint32_t)c : 0u;
+  B->rs = reinterpret_cast<uint64_t*>(take(c * 8));
+  B->ri = reinterpret_cast<uint64_t*>(take(c * 8));
+  B->q[0] = reinterpret_cast<uint32_t*>(take(c * 4));
+  B->q[1] = reinterpret_cast<uint32_t*>(take(c * 4));
+  B->hprim = reinterpret_cast<uint32_t*>(take(c * 4));
+  B->hplane = reinterpret_cast<int32_t*>(take(c * 4));
+  B->sq_pid = reinterpret_cast<uint32_t*>(take(sq * 4));
+  B->spec = reinterpret_cast<uint8_t*>(take(c));
+  B->counts = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * n_counts));
+  B->stats = batch ? reinterpret_cast<unsigned long long*>(take(8 * n_stats)) : stats;
+  B->capacity = cap;
+  B->sub_cap = sub_cap;
+  return bytes;
+}
+
+// Grows the context's path state `which` to room for `cap` paths.
 template <class T>
 rtg_status ensure_paths(rtg_context* ctx, int which, uint32_t cap, PathBuffers<T>& B) {
   if (ctx->cap[which] >= cap) return RTG_OK;
-  if (which == kBatchBuffers) drop_batch_graphs(ctx);
+  const bool batch = which == kBatchBuffers;
+  if (batch) drop_batch_graphs(ctx);
   if (ctx->path_mem[which]) {
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream2));
@@ -755,53 +845,10 @@ rtg_status ensure_paths(rtg_context* ctx, int which, uint32_t cap, PathBuffers<T
     ctx->path_mem[which] = nullptr;
     ctx->cap[which] = 0;
   }
-  const uint32_t sub_cap = sub_queue_cap(cap);
-  const uint64_t c = which == kBatchBuffers ? (uint64_t)cap + kBatchQueueSlack
-                                            : (uint64_t)sub_cap * kSubQueues;  // >= cap
-  auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
-  const uint64_t tbytes = al(c * sizeof(T));
-  // 6 ray + 3 beta + 3 L + 3 hit + 10 shadow arrays of T; 2 RNG u64; 5 u32
-  // (2 queues, hprim, hplane, sq_pid); spec; counts. FP32: a second shadow
-  // queue (10 T + sq_pid) and 3 shadow-radiance arrays.
-  constexpr bool f32 = sizeof(T) == 4;
-  const uint64_t sq = f32 ? 2 * c : c;  // shadow-queue entries
-  const bool batch = which == kBatchBuffers;
-  const uint64_t n_counts = batch ? (uint64_t)kMaxBatchJobs * kBatchCounts : 2ull * kMaxBounce * kSubQueues;
-  const uint64_t n_stats = batch ? (uint64_t)kMaxBatchJobs * kBatchStatSlots * kStatStride : 0;
-  const uint64_t bytes = tbytes * (15 + (f32 ? 3 : 0)) + al(sq * sizeof(T)) * 10 + al(c * 8) * 2 + al(c * 4) * 4 +
-                         al(sq * 4) + al(c) + al(sizeof(uint32_t) * n_counts) + al(8 * n_stats);
+  const uint64_t bytes = lay_out_paths<T>(nullptr, cap, batch, ctx->stats, nullptr);
   void* mem = nullptr;
   HIP_TRY(ctx, hipMalloc(&mem, bytes));
-  char* p = static_cast<char*>(mem);
-  auto take = [&](uint64_t b) {
-    char* r = p;
-    p += al(b);
-    return r;
-  };
-  T** tarr[] = {&B.ox, &B.oy, &B.oz, &B.dx, &B.dy, &B.dz, &B.bx, &B.by, &B.bz, &B.lx, &B.ly, &B.lz, &B.ht,
-                &B.hb1, &B.hb2};
-  for (T** a : tarr) *a = reinterpret_cast<T*>(take(c * sizeof(T)));
-  T** sarr[] = {&B.sox, &B.soy, &B.soz, &B.sdx, &B.sdy, &B.sdz, &B.stm, &B.scx, &B.scy, &B.scz};
-  for (T** a : sarr) *a = reinterpret_cast<T*>(take(sq * sizeof(T)));
-  B.slx = B.sly = B.slz = nullptr;
-  if (f32) {
-    B.slx = reinterpret_cast<T*>(take(c * sizeof(T)));
-    B.sly = reinterpret_cast<T*>(take(c * sizeof(T)));
-    B.slz = reinterpret_cast<T*>(take(c * sizeof(T)));
-  }
-  B.sq_stride = f32 ? (uint32_t)c : 0u;
-  B.rs = reinterpret_cast<uint64_t*>(take(c * 8));
-  B.ri = reinterpret_cast<uint64_t*>(take(c * 8));
-  B.q[0] = reinterpret_cast<uint32_t*>(take(c * 4));
-  B.q[1] = reinterpret_cast<uint32_t*>(take(c * 4));
-  B.hprim = reinterpret_cast<uint32_t*>(take(c * 4));
-  B.hplane = reinterpret_cast<int32_t*>(take(c * 4));
-  B.sq_pid = reinterpret_cast<uint32_t*>(take(sq * 4));
-  B.spec = reinterpret_cast<uint8_t*>(take(c));
-  B.counts = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * n_counts));
-  B.stats = batch ? reinterpret_cast<unsigned long long*>(take(8 * n_stats)) : ctx->stats;
-  B.capacity = cap;
-  B.sub_cap = sub_cap;
+  lay_out_paths<T>(static_cast<char*>(mem), cap, batch, ctx->stats, &B);
   ctx->path_mem[which] = mem;
   ctx->path_bytes[which] = bytes;
   ctx->cap[which] = cap;
@@ -855,17 +902,18 @@ void read_stats(rtg_context* ctx, rtg_stats* out, float ms) {
   out->elapsed_ms = ms;
 }
 
-rtg_status check_settings(rtg_context* ctx, const std::string& fn, const rtg_render_settings* cfg,
+rtg_status check_settings(rtg_context* ctx, const char* fn_name, const rtg_render_settings* cfg,
                           const rtg_accum* acc, uint32_t s0, uint32_t s1) {
+  auto fn = [&]() { return std::string(fn_name); };
   if (cfg->width != acc->w || cfg->height != acc->h)
-    return fail(ctx, RTG_ERR_INVALID_ARG, fn + ": settings size differs from the accumulation buffer");
-  if (s1 <= s0) return fail(ctx, RTG_ERR_INVALID_ARG, fn + ": empty sample range");
+    return fail(ctx, RTG_ERR_INVALID_ARG, fn() + ": settings size differs from the accumulation buffer");
+  if (s1 <= s0) return fail(ctx, RTG_ERR_INVALID_ARG, fn() + ": empty sample range");
   if (cfg->max_depth < 1 || cfg->max_depth >= kMaxBounce)
-    return fail(ctx, RTG_ERR_INVALID_ARG, fn + ": max_depth must be in [1, 1023]");
+    return fail(ctx, RTG_ERR_INVALID_ARG, fn() + ": max_depth must be in [1, 1023]");
   if (cfg->integrator < 0 || cfg->integrator > 2 || cfg->precision < 0 || cfg->precision > 1)
-    return fail(ctx, RTG_ERR_INVALID_ARG, fn + ": bad integrator or precision");
+    return fail(ctx, RTG_ERR_INVALID_ARG, fn() + ": bad integrator or precision");
   if (cfg->integrator == RTG_INTEGRATOR_DIRECT && cfg->max_depth > 31)
-    return fail(ctx, RTG_ERR_UNSUPPORTED, fn + ": the direct integrator supports max_depth <= 31");
+    return fail(ctx, RTG_ERR_UNSUPPORTED, fn() + ": the direct integrator supports max_depth <= 31");
   return RTG_OK;
 }
 
@@ -893,6 +941,17 @@ uint32_t capacity_for(uint64_t paths) {
   uint32_t cap = 1024;
   while (cap < paths && cap < lim) cap <<= 1;
   return std::min(cap, lim);
+}
+
+// The largest chunk (a power of two, at least 1024 paths) whose path state
+// fits `bytes`; 0 if none does.
+uint32_t static_capacity(bool f64, uint64_t bytes) {
+  for (uint32_t cap = 1u << 25; cap >= 1024; cap >>= 1) {
+    const uint64_t need = f64 ? lay_out_paths<double>(nullptr, cap, false, nullptr, nullptr)
+                              : lay_out_paths<float>(nullptr, cap, false, nullptr, nullptr);
+    if (need <= bytes) return cap;
+  }
+  return 0;
 }
 
 // Chunks of a path-integrator pass on two lanes (passes of at least
@@ -1223,6 +1282,81 @@ rtg_status rtg_render_passes(rtg_context* ctx, const rtg_scene* scene, const rtg
                        passes_done, stats, "rtg_render_passes");
 }
 
+rtg_status rtg_render_pass_static(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* cfg,
+                                  uint32_t s0, uint32_t s1, rtg_accum* acc, const volatile int32_t* cancel,
+                                  rtg_stats* stats) {
+  static const char* const name = "rtg_render_pass_static";
+  auto fn = [&]() { return std::string(name); };  // built only for an error message
+  if (!ctx || !scene || !cfg || !acc) return fail(ctx, RTG_ERR_INVALID_ARG, fn() + ": NULL argument");
+  if (rtg_status st = check_settings(ctx, name, cfg, acc, s0, s1)) return st;
+  if (!ctx->scratch_dev) return fail(ctx, RTG_ERR_HIP, fn() + ": the device scratch array was not set up");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const bool f64 = cfg->precision == RTG_PRECISION_F64;
+  const uint32_t n_pix = (uint32_t)acc->w * (uint32_t)acc->h;
+  const uint64_t paths = (uint64_t)(s1 - s0) * n_pix;
+  // The scratch holds the statistics slots, then one chunk's path state.
+  const uint32_t n_slots = kStatSlots * kStatStride;
+  const uint64_t slot_bytes = ((uint64_t)n_slots * 8 + 255) & ~uint64_t(255);
+  unsigned long long* slots = reinterpret_cast<unsigned long long*>(ctx->scratch_dev);
+  char* base = ctx->scratch_dev + slot_bytes;
+  const uint32_t cap = std::min(capacity_for(paths), static_capacity(f64, kStaticScratchBytes - slot_bytes));
+  if (cap == 0) return fail(ctx, RTG_ERR_UNSUPPORTED, fn() + ": the scratch array holds no chunk");
+  PathBuffers<float> B32{};
+  PathBuffers<double> B64{};
+  const uint64_t need = f64 ? lay_out_paths<double>(base, cap, false, slots, &B64)
+                            : lay_out_paths<float>(base, cap, false, slots, &B32);
+  if (need + slot_bytes > kStaticScratchBytes)
+    return fail(ctx, RTG_ERR_OUT_OF_MEMORY, fn() + ": chunk state exceeds the scratch array");
+  hipLaunchKernelGGL(k_zero_u64, dim3((n_slots + 127) / 128), dim3(128), 0, ctx->stream, slots, n_slots);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  // One lane: chunks of whole pixels x a sample sub-range, in sample order
+  // per pixel (the chunking of rtg_render_pass, bitwise the same result).
+  bool cancelled = false, used = false;
+  for (uint32_t sa = s0; sa < s1 && !cancelled;) {
+    const uint32_t k = std::min(s1 - sa, cap);
+    const uint32_t pix_per_chunk = std::max(1u, cap / k);
+    for (uint32_t p0 = 0; p0 < n_pix; p0 += pix_per_chunk) {
+      if (cancel) {
+        if (used) HIP_TRY(ctx, hipEventSynchronize(ctx->acc_done[0]));
+        if (*cancel) {
+          cancelled = true;
+          break;
+        }
+      }
+      ChunkParams P = render_params(cfg, acc);
+      P.p0 = p0;
+      P.k = k;
+      P.s0 = sa;
+      P.n_pix = std::min(pix_per_chunk, n_pix - p0);
+      P.n_paths = P.n_pix * k;
+      P.sub_cap = f64 ? B64.sub_cap : B32.sub_cap;
+      P.shadow_counts = kMaxBounce * kSubQueues;
+      P.stat_slots = kStatSlots;
+      P.counts = f64 ? B64.counts : B32.counts;
+      P.stats = slots;
+      const hipEvent_t rec = cancel ? ctx->acc_done[0] : nullptr;
+      if (f64)
+        run_chunk(scene->dev, B64, P, scene->stack, ctx->stream, side_of(ctx), nullptr, nullptr, rec);
+      else
+        run_chunk(scene->dev, B32, P, scene->stack, ctx->stream, side_of(ctx), nullptr, nullptr, rec);
+      HIP_TRY(ctx, hipGetLastError());
+      used = true;
+    }
+    sa += k;
+  }
+  if (!stats && !cancel) return RTG_OK;  // enqueued on the context's stream
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->stats_host, slots, sizeof(unsigned long long) * n_slots, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  read_stats(ctx, stats, ms);
+  if (cancelled) return fail(ctx, RTG_ERR_CANCELLED, fn() + ": cancelled");
+  return RTG_OK;
+}
+
 rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jobs, rtg_stats* stats) {
   if (!ctx || (n_jobs && !jobs)) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_batch: NULL argument");
   const std::string fn = "rtg_render_batch";
@@ -1231,7 +1365,7 @@ rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jo
     const rtg_job& J = jobs[i];
     const std::string fi = fn + ": job " + std::to_string(i);
     if (!J.scene || !J.acc) return fail(ctx, RTG_ERR_INVALID_ARG, fi + ": NULL scene or accumulation buffer");
-    if (rtg_status st = check_settings(ctx, fi, &J.settings, J.acc, J.s0, J.s1)) return st;
+    if (rtg_status st = check_settings(ctx, fi.c_str(), &J.settings, J.acc, J.s0, J.s1)) return st;
     const uint32_t n_pix = (uint32_t)J.acc->w * (uint32_t)J.acc->h;
     p0[i] = J.p1 == 0 ? 0 : J.p0;
     p1[i] = J.p1 == 0 ? n_pix : J.p1;

@@ -148,6 +148,8 @@ def load_rtg():
         "rtg_render_passes": (C.c_int32, [_VP, _VP, P(RenderSettings), P(C.c_uint32), C.c_uint32, _VP, _VP,
                                           P(C.c_uint32), P(Stats)]),
         "rtg_render_batch": (C.c_int32, [_VP, P(Job), C.c_uint32, P(Stats)]),
+        "rtg_render_pass_static": (C.c_int32, [_VP, _VP, P(RenderSettings), C.c_uint32, C.c_uint32, _VP, _VP,
+                                               _VP]),
         "rtg_resolve": (C.c_int32, [_VP, _VP, C.c_int32, C.c_double, _VP]),
         "rtg_buffer_create": (C.c_int32, [_VP, C.c_uint64, P(_VP)]),
         "rtg_buffer_release": (C.c_int32, [_VP]),
@@ -170,7 +172,7 @@ def rtg_symbols():
             "rtg_scene_node_records",
             "rtg_scene_release", "rtg_scene_with_camera", "rtg_scene_bytes", "rtg_accum_create", "rtg_accum_release", "rtg_accum_clear",
             "rtg_accum_read", "rtg_accum_min_samples", "rtg_render_pass", "rtg_render_passes", "rtg_render_batch",
-            "rtg_resolve",
+            "rtg_render_pass_static", "rtg_resolve",
             "rtg_buffer_create",
             "rtg_buffer_release", "rtg_buffer_write", "rtg_buffer_read", "rtg_trace", "rtg_trace_host",
             "rtg_synchronize", "rtg_set_profiling", "rtg_set_profiling_kinds", "rtg_get_profile"]
@@ -239,6 +241,20 @@ class Context:
         st = Stats()
         cp = C.cast(C.pointer(cancel), C.c_void_p) if cancel is not None else None
         rc = self.lib.rtg_render_pass(self.h, scene.h, C.byref(settings), s0, s1, acc.h, cp, C.byref(st))
+        if rc == -4:
+            return st, True
+        self._check(rc)
+        return st, False
+
+    def render_pass_static(self, scene, settings: RenderSettings, s0, s1, acc, cancel=None, stats=True):
+        """rtg_render_pass_static: the pass with no device allocation (path
+        state in the library's fixed device scratch). stats=False passes NULL
+        (the call only enqueues when cancel is None too). Returns (Stats or
+        None, cancelled)."""
+        st = Stats() if stats else None
+        cp = C.cast(C.pointer(cancel), C.c_void_p) if cancel is not None else None
+        rc = self.lib.rtg_render_pass_static(self.h, scene.h, C.byref(settings), s0, s1, acc.h, cp,
+                                             C.byref(st) if st is not None else None)
         if rc == -4:
             return st, True
         self._check(rc)
