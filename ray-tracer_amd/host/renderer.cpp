@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <stdexcept>
+#include <thread>
 
 namespace rtm {
 
@@ -89,11 +90,46 @@ void GpuRenderer::add_stats(const rtg_stats& st, uint32_t passes) {
   fb_.valid_ = false;
 }
 
-RenderProgress GpuRenderer::render(const ProgressFn& on_pass, const std::atomic<int32_t>* cancel) {
+namespace {
+// The device driver polls a plain int flag between chunks (rtg.h); a
+// caller's std::atomic<bool> request is copied into one by a watcher thread
+// for the duration of a render.
+class CancelMirror {
+ public:
+  explicit CancelMirror(const std::atomic<bool>* request) : request_(request) {
+    if (!request_) return;
+    watcher_ = std::thread([this] {
+      while (!done_.load()) {
+        if (request_->load()) {
+          raised_.store(1);
+          return;
+        }
+        std::this_thread::sleep_for(std::chrono::microseconds(200));
+      }
+    });
+  }
+  ~CancelMirror() {
+    done_.store(true);
+    if (watcher_.joinable()) watcher_.join();
+  }
+  const volatile int32_t* flag() const {
+    return request_ ? reinterpret_cast<const volatile int32_t*>(&raised_) : nullptr;
+  }
+
+ private:
+  const std::atomic<bool>* request_;
+  std::atomic<bool> done_{false};
+  std::atomic<int32_t> raised_{0};
+  std::thread watcher_;
+};
+}  // namespace
+
+RenderProgress GpuRenderer::render(const ProgressFn& on_pass, const std::atomic<bool>* cancel) {
   const auto t0 = std::chrono::steady_clock::now();
   const uint32_t target = (uint32_t)scene_.settings.spp;
   const rtg_render_settings cfg = scene_.device_settings(opts_.precision);
-  const volatile int32_t* flag = reinterpret_cast<const volatile int32_t*>(cancel);
+  const CancelMirror mirror(cancel);
+  const volatile int32_t* flag = mirror.flag();
   RenderProgress prog;
   prog.spp_target = scene_.settings.spp;
   uint32_t done = 0;
@@ -178,7 +214,7 @@ RenderSession::~RenderSession() {
 
 void RenderSession::start(ProgressFn on_pass) {
   wait();  // one job at a time
-  cancel_.store(0);
+  cancel_.store(false);
   error_.clear();
   running_.store(true);
   renderer_.reset();

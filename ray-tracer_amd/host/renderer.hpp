@@ -85,7 +85,9 @@ class GpuRenderer {
   GpuRenderer(const GpuRenderer&) = delete;
   GpuRenderer& operator=(const GpuRenderer&) = delete;
 
-  RenderProgress render(const ProgressFn& on_pass = {}, const std::atomic<int32_t>* cancel = nullptr);
+  // `cancel` may be raised from any thread (renderer.hpp:56 of the
+  // reference); it is polled between device chunks.
+  RenderProgress render(const ProgressFn& on_pass = {}, const std::atomic<bool>* cancel = nullptr);
 
   const Framebuffer& framebuffer() const { return fb_; }
   Image8 image() const;  // device tone map + sRGB
@@ -116,7 +118,7 @@ class RenderSession {
   RenderSession& operator=(const RenderSession&) = delete;
 
   void start(ProgressFn on_pass = {});
-  void cancel() { cancel_.store(1); }
+  void cancel() { cancel_.store(true); }
   RenderProgress wait();
   void restart(CompiledScene scene, ProgressFn on_pass = {});
   bool running() const { return running_.load(); }
@@ -127,7 +129,7 @@ class RenderSession {
   CompiledScene scene_;
   RenderOptions opts_;
   std::unique_ptr<GpuRenderer> renderer_;
-  std::atomic<int32_t> cancel_{0};
+  std::atomic<bool> cancel_{false};
   std::atomic<bool> running_{false};
   RenderProgress last_;
   std::string error_;

@@ -28,147 +28,155 @@ const char* tone_map_name(ToneMap t) {
 
 namespace {
 
-// Typed field access relative to a JSON path ("objects[2]", "render", ...).
-class Fields {
- public:
-  Fields(const Json& j, std::string path) : j_(j), path_(std::move(path)) {
-    if (!j_.is_object()) throw SceneError((path_.empty() ? std::string("scene") : path_) + ": expected an object");
+// A JSON object being read, and its path in the document ("objects[2]",
+// "render", "" for the root) for error messages.
+struct Scope {
+  Scope(const Json& j, std::string p) : node(j), path(std::move(p)) {
+    if (!node.is_object()) throw SceneError((path.empty() ? std::string("scene") : path) + ": expected an object");
   }
-  std::string at(const std::string& k) const { return path_.empty() ? k : path_ + "." + k; }
-  const Json* get(const std::string& k) const { return j_.get(k); }
-  const std::string& path() const { return path_; }
+  std::string where(const std::string& key) const { return path.empty() ? key : path + "." + key; }
+  const Json* find(const std::string& key) const { return node.get(key); }
+  const Json& node;
+  std::string path;
+};
 
-  double num(const std::string& k, double def) const {
-    const Json* v = get(k);
-    if (!v) return def;
-    if (!v->is_number()) throw SceneError(at(k) + ": expected a number");
-    return v->as_number();
+// Conversion of a present value to T; `at` names it in the error.
+template <class T>
+struct As;
+template <>
+struct As<double> {
+  static double from(const Json& v, const std::string& at) {
+    if (!v.is_number()) throw SceneError(at + ": expected a number");
+    return v.as_number();
   }
-  int integer(const std::string& k, int def) const {
-    const Json* v = get(k);
-    if (!v) return def;
-    if (!v->is_number()) throw SceneError(at(k) + ": expected an integer");
-    const double d = v->as_number();
-    if (d != std::floor(d) || std::fabs(d) > 2e9) throw SceneError(at(k) + ": expected an integer");
-    return (int)d;
+};
+template <>
+struct As<int> {  // a whole number of magnitude at most 2e9
+  static int from(const Json& v, const std::string& at) {
+    if (!v.is_number() || v.as_number() != std::floor(v.as_number()) || std::fabs(v.as_number()) > 2e9)
+      throw SceneError(at + ": expected an integer");
+    return (int)v.as_number();
   }
-  bool flag(const std::string& k, bool def) const {
-    const Json* v = get(k);
-    if (!v) return def;
-    if (!v->is_bool()) throw SceneError(at(k) + ": expected true/false");
-    return v->as_bool();
+};
+template <>
+struct As<bool> {
+  static bool from(const Json& v, const std::string& at) {
+    if (!v.is_bool()) throw SceneError(at + ": expected true/false");
+    return v.as_bool();
   }
-  std::string str(const std::string& k, const std::string& def) const {
-    const Json* v = get(k);
-    if (!v) return def;
-    if (!v->is_string()) throw SceneError(at(k) + ": expected a string");
-    return v->as_string();
+};
+template <>
+struct As<std::string> {
+  static std::string from(const Json& v, const std::string& at) {
+    if (!v.is_string()) throw SceneError(at + ": expected a string");
+    return v.as_string();
   }
-  std::string need_str(const std::string& k) const {
-    if (!get(k)) throw SceneError(at(k) + ": required");
-    return str(k, "");
-  }
-  Vec3d vec(const std::string& k, Vec3d def) const {
-    const Json* v = get(k);
-    if (!v) return def;
-    if (v->is_number()) return {v->as_number(), v->as_number(), v->as_number()};
-    if (!v->is_array() || v->items().size() != 3) throw SceneError(at(k) + ": expected [x, y, z]");
+};
+template <>
+struct As<Vec3d> {  // [x, y, z], or one number for all three
+  static Vec3d from(const Json& v, const std::string& at) {
+    if (v.is_number()) return {v.as_number(), v.as_number(), v.as_number()};
+    if (!v.is_array() || v.items().size() != 3) throw SceneError(at + ": expected [x, y, z]");
     double c[3];
-    for (int i = 0; i < 3; ++i) {
-      const Json& e = v->items()[(size_t)i];
-      if (!e.is_number()) throw SceneError(at(k) + ": expected [x, y, z] of numbers");
-      c[i] = e.as_number();
+    for (size_t i = 0; i < 3; ++i) {
+      if (!v.items()[i].is_number()) throw SceneError(at + ": expected [x, y, z] of numbers");
+      c[i] = v.items()[i].as_number();
     }
     return {c[0], c[1], c[2]};
   }
-  Vec3d need_vec(const std::string& k) const {
-    if (!get(k)) throw SceneError(at(k) + ": required");
-    return vec(k, {});
-  }
-
- private:
-  const Json& j_;
-  std::string path_;
 };
 
-MaterialDesc read_material(const std::string& name, const Fields& f) {
+// An optional field (the fallback when absent) and a required one.
+template <class T>
+T opt(const Scope& s, const std::string& key, const T& fallback) {
+  const Json* v = s.find(key);
+  return v ? As<T>::from(*v, s.where(key)) : fallback;
+}
+template <class T>
+T req(const Scope& s, const std::string& key) {
+  const Json* v = s.find(key);
+  if (!v) throw SceneError(s.where(key) + ": required");
+  return As<T>::from(*v, s.where(key));
+}
+
+MaterialDesc read_material(const std::string& name, const Scope& f) {
   MaterialDesc m;
   m.name = name;
-  const std::string type = f.need_str("type");
+  const std::string type = req<std::string>(f, "type");
   if (type == "lambertian" || type == "diffuse") {
     m.kind = RTG_MAT_LAMBERTIAN;
-    m.c0 = f.vec("albedo", {0.5, 0.5, 0.5});
-    m.c1 = f.vec("albedo2", m.c0);
-    m.p0 = f.num("checker_scale", 0);
+    m.c0 = opt<Vec3d>(f, "albedo", {0.5, 0.5, 0.5});
+    m.c1 = opt<Vec3d>(f, "albedo2", m.c0);
+    m.p0 = opt<double>(f, "checker_scale", 0);
   } else if (type == "metal") {
     m.kind = RTG_MAT_METAL;
-    m.c0 = f.vec("albedo", {0.9, 0.9, 0.9});
+    m.c0 = opt<Vec3d>(f, "albedo", {0.9, 0.9, 0.9});
     m.c1 = {};
-    m.p0 = f.num("roughness", 0);
+    m.p0 = opt<double>(f, "roughness", 0);
   } else if (type == "dielectric" || type == "glass") {
     m.kind = RTG_MAT_DIELECTRIC;
-    m.p0 = f.num("ior", 1.5);
-    m.c0 = f.vec("tint", {1, 1, 1});
+    m.p0 = opt<double>(f, "ior", 1.5);
+    m.c0 = opt<Vec3d>(f, "tint", {1, 1, 1});
     m.c1 = {};
   } else if (type == "emissive" || type == "light") {
     m.kind = RTG_MAT_EMISSIVE;
-    m.c0 = f.vec("color", {1, 1, 1});
+    m.c0 = opt<Vec3d>(f, "color", {1, 1, 1});
     m.c1 = {};
-    m.p0 = f.num("intensity", 1);
-    m.two_sided = f.flag("two_sided", false);
+    m.p0 = opt<double>(f, "intensity", 1);
+    m.two_sided = opt<bool>(f, "two_sided", false);
   } else {
-    throw SceneError(f.at("type") + ": unknown material type '" + type + "'");
+    throw SceneError(f.where("type") + ": unknown material type '" + type + "'");
   }
   return m;
 }
 
-ObjectDesc read_object(const Fields& f) {
+ObjectDesc read_object(const Scope& f) {
   ObjectDesc o;
-  const std::string type = f.need_str("type");
-  o.name = f.str("name", type);
-  o.material = f.need_str("material");
+  const std::string type = req<std::string>(f, "type");
+  o.name = opt<std::string>(f, "name", type);
+  o.material = req<std::string>(f, "material");
   if (type == "sphere") {
     o.shape = Shape::Sphere;
-    o.a = f.need_vec("center");
-    o.radius = f.num("radius", 1);
+    o.a = req<Vec3d>(f, "center");
+    o.radius = opt<double>(f, "radius", 1);
   } else if (type == "plane") {
     o.shape = Shape::Plane;
-    o.a = f.vec("point", {});
-    o.b = f.vec("normal", {0, 1, 0});
+    o.a = opt<Vec3d>(f, "point", {});
+    o.b = opt<Vec3d>(f, "normal", {0, 1, 0});
   } else if (type == "triangle") {
     o.shape = Shape::Triangle;
-    o.a = f.need_vec("v0");
-    o.b = f.need_vec("v1");
-    o.c = f.need_vec("v2");
+    o.a = req<Vec3d>(f, "v0");
+    o.b = req<Vec3d>(f, "v1");
+    o.c = req<Vec3d>(f, "v2");
   } else if (type == "quad") {
     o.shape = Shape::Quad;
-    o.a = f.need_vec("corner");
-    o.b = f.need_vec("u");
-    o.c = f.need_vec("v");
+    o.a = req<Vec3d>(f, "corner");
+    o.b = req<Vec3d>(f, "u");
+    o.c = req<Vec3d>(f, "v");
   } else if (type == "box") {
     o.shape = Shape::Box;
-    o.a = f.need_vec("min");
-    o.b = f.need_vec("max");
-    o.rotate_y = f.num("rotate_y", 0);
+    o.a = req<Vec3d>(f, "min");
+    o.b = req<Vec3d>(f, "max");
+    o.rotate_y = opt<double>(f, "rotate_y", 0);
   } else if (type == "mesh") {
     o.shape = Shape::Mesh;
-    const std::string src = f.str("source", "icosphere");
+    const std::string src = opt<std::string>(f, "source", "icosphere");
     if (src == "icosphere") o.source = MeshSource::Icosphere;
     else if (src == "torus") o.source = MeshSource::Torus;
     else if (src == "obj") o.source = MeshSource::Obj;
-    else throw SceneError(f.at("source") + ": expected icosphere, torus or obj");
-    o.subdivisions = f.integer("subdivisions", o.subdivisions);
-    o.major_radius = f.num("major_radius", o.major_radius);
-    o.minor_radius = f.num("minor_radius", o.minor_radius);
-    o.segments = f.integer("segments", o.segments);
-    o.segments_v = f.integer("segments_v", 0);
-    o.path = f.str("path", "");
-    o.smooth = f.flag("smooth", true);
-    o.scale = f.vec("scale", {1, 1, 1});
-    o.rotate_y = f.num("rotate_y", 0);
-    o.translate = f.vec("translate", {0, 0, 0});
+    else throw SceneError(f.where("source") + ": expected icosphere, torus or obj");
+    o.subdivisions = opt<int>(f, "subdivisions", o.subdivisions);
+    o.major_radius = opt<double>(f, "major_radius", o.major_radius);
+    o.minor_radius = opt<double>(f, "minor_radius", o.minor_radius);
+    o.segments = opt<int>(f, "segments", o.segments);
+    o.segments_v = opt<int>(f, "segments_v", 0);
+    o.path = opt<std::string>(f, "path", "");
+    o.smooth = opt<bool>(f, "smooth", true);
+    o.scale = opt<Vec3d>(f, "scale", {1, 1, 1});
+    o.rotate_y = opt<double>(f, "rotate_y", 0);
+    o.translate = opt<Vec3d>(f, "translate", {0, 0, 0});
   } else {
-    throw SceneError(f.at("type") + ": unknown object type '" + type + "'");
+    throw SceneError(f.where("type") + ": unknown object type '" + type + "'");
   }
   return o;
 }
@@ -185,7 +193,7 @@ Json vj(const Vec3d& v) {
 
 SceneDesc scene_from_json(const Json& root) {
   SceneDesc s;
-  const Fields r(root, "");
+  const Scope r(root, "");
   static const char* known[] = {"name", "camera", "render", "background", "materials", "objects", "lights", "$schema",
                                 "description"};
   for (const auto& kv : root.members()) {
@@ -193,72 +201,72 @@ SceneDesc scene_from_json(const Json& root) {
     for (const char* k : known) ok = ok || kv.first == k;
     if (!ok) throw SceneError("scene: unknown top-level key '" + kv.first + "'");
   }
-  s.name = r.str("name", s.name);
-  if (const Json* cj = r.get("camera")) {
-    const Fields c(*cj, "camera");
-    s.camera.look_from = c.vec("look_from", s.camera.look_from);
-    s.camera.look_at = c.vec("look_at", s.camera.look_at);
-    s.camera.vup = c.vec("vup", s.camera.vup);
-    s.camera.vfov_deg = c.num("vfov", s.camera.vfov_deg);
-    s.camera.aperture = c.num("aperture", 0);
-    s.camera.focus_dist = c.num("focus_dist", 0);
+  s.name = opt<std::string>(r, "name", s.name);
+  if (const Json* cj = r.find("camera")) {
+    const Scope c(*cj, "camera");
+    s.camera.look_from = opt<Vec3d>(c, "look_from", s.camera.look_from);
+    s.camera.look_at = opt<Vec3d>(c, "look_at", s.camera.look_at);
+    s.camera.vup = opt<Vec3d>(c, "vup", s.camera.vup);
+    s.camera.vfov_deg = opt<double>(c, "vfov", s.camera.vfov_deg);
+    s.camera.aperture = opt<double>(c, "aperture", 0);
+    s.camera.focus_dist = opt<double>(c, "focus_dist", 0);
   }
-  if (const Json* rj = r.get("render")) {
-    const Fields c(*rj, "render");
+  if (const Json* rj = r.find("render")) {
+    const Scope c(*rj, "render");
     RenderSettings& R = s.render;
-    R.width = c.integer("width", R.width);
-    R.height = c.integer("height", R.height);
-    R.spp = c.integer("spp", R.spp);
-    R.max_depth = c.integer("max_depth", R.max_depth);
-    R.rr_min_depth = c.integer("rr_min_depth", R.rr_min_depth);
-    R.light_sampling = c.flag("light_sampling", R.light_sampling);
-    R.seed = (uint64_t)c.integer("seed", (int)R.seed);
-    R.exposure = c.num("exposure", R.exposure);
-    R.threads = c.integer("threads", R.threads);
-    R.tile_size = c.integer("tile_size", R.tile_size);
-    R.use_bvh = c.flag("use_bvh", R.use_bvh);
-    const std::string integ = c.str("integrator", integrator_name(R.integrator));
+    R.width = opt<int>(c, "width", R.width);
+    R.height = opt<int>(c, "height", R.height);
+    R.spp = opt<int>(c, "spp", R.spp);
+    R.max_depth = opt<int>(c, "max_depth", R.max_depth);
+    R.rr_min_depth = opt<int>(c, "rr_min_depth", R.rr_min_depth);
+    R.light_sampling = opt<bool>(c, "light_sampling", R.light_sampling);
+    R.seed = (uint64_t)opt<int>(c, "seed", (int)R.seed);
+    R.exposure = opt<double>(c, "exposure", R.exposure);
+    R.threads = opt<int>(c, "threads", R.threads);
+    R.tile_size = opt<int>(c, "tile_size", R.tile_size);
+    R.use_bvh = opt<bool>(c, "use_bvh", R.use_bvh);
+    const std::string integ = opt<std::string>(c, "integrator", integrator_name(R.integrator));
     if (integ == "path") R.integrator = Integrator::Path;
     else if (integ == "direct") R.integrator = Integrator::Direct;
     else if (integ == "normals") R.integrator = Integrator::Normals;
     else throw SceneError("render.integrator: expected path, direct or normals");
-    const std::string tm = c.str("tone_map", tone_map_name(R.tone_map));
+    const std::string tm = opt<std::string>(c, "tone_map", tone_map_name(R.tone_map));
     if (tm == "clamp") R.tone_map = ToneMap::Clamp;
     else if (tm == "reinhard") R.tone_map = ToneMap::Reinhard;
     else if (tm == "aces") R.tone_map = ToneMap::Aces;
     else throw SceneError("render.tone_map: expected clamp, reinhard or aces");
   }
-  if (const Json* bj = r.get("background")) {
-    const Fields b(*bj, "background");
-    const std::string kind = b.str("type", "gradient");
+  if (const Json* bj = r.find("background")) {
+    const Scope b(*bj, "background");
+    const std::string kind = opt<std::string>(b, "type", "gradient");
     if (kind == "gradient") s.background.solid = false;
     else if (kind == "solid") s.background.solid = true;
     else throw SceneError("background.type: expected gradient or solid");
-    s.background.top = b.vec(kind == "solid" ? "color" : "top", s.background.top);
-    s.background.bottom = b.vec("bottom", s.background.bottom);
-    s.background.intensity = b.num("intensity", 1);
+    s.background.top = opt<Vec3d>(b, kind == "solid" ? "color" : "top", s.background.top);
+    s.background.bottom = opt<Vec3d>(b, "bottom", s.background.bottom);
+    s.background.intensity = opt<double>(b, "intensity", 1);
   }
-  if (const Json* mj = r.get("materials")) {
+  if (const Json* mj = r.find("materials")) {
     if (!mj->is_object()) throw SceneError("materials: expected an object of name -> material");
-    for (const auto& kv : mj->members()) s.materials.push_back(read_material(kv.first, Fields(kv.second, "materials." + kv.first)));
+    for (const auto& kv : mj->members()) s.materials.push_back(read_material(kv.first, Scope(kv.second, "materials." + kv.first)));
   }
-  if (const Json* oj = r.get("objects")) {
+  if (const Json* oj = r.find("objects")) {
     if (!oj->is_array()) throw SceneError("objects: expected an array");
     const auto& a = oj->items();
-    for (size_t i = 0; i < a.size(); ++i) s.objects.push_back(read_object(Fields(a[i], "objects[" + std::to_string(i) + "]")));
+    for (size_t i = 0; i < a.size(); ++i) s.objects.push_back(read_object(Scope(a[i], "objects[" + std::to_string(i) + "]")));
   }
-  if (const Json* lj = r.get("lights")) {
+  if (const Json* lj = r.find("lights")) {
     if (!lj->is_array()) throw SceneError("lights: expected an array");
     const auto& a = lj->items();
     for (size_t i = 0; i < a.size(); ++i) {
-      const Fields l(a[i], "lights[" + std::to_string(i) + "]");
-      const std::string type = l.str("type", "point");
+      const Scope l(a[i], "lights[" + std::to_string(i) + "]");
+      const std::string type = opt<std::string>(l, "type", "point");
       if (type != "point")
-        throw SceneError(l.at("type") + ": only 'point' lights are supported (use emissive materials for area lights)");
+        throw SceneError(l.where("type") + ": only 'point' lights are supported (use emissive materials for area lights)");
       PointLightDesc p;
-      p.name = l.str("name", "light" + std::to_string(i));
-      p.position = l.need_vec("position");
-      p.intensity = l.vec("intensity", {10, 10, 10});
+      p.name = opt<std::string>(l, "name", "light" + std::to_string(i));
+      p.position = req<Vec3d>(l, "position");
+      p.intensity = opt<Vec3d>(l, "intensity", {10, 10, 10});
       s.lights.push_back(p);
     }
   }

@@ -33,6 +33,7 @@
 
 #include "image_io.hpp"
 #include "json.hpp"
+#include "options.hpp"
 #include "renderer.hpp"
 #include "scene.hpp"
 #include "tick_tuner.hpp"
@@ -45,8 +46,10 @@ using Clock = std::chrono::steady_clock;
 double ms_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
-std::string jstr(const std::string& s) { return Json::string(s).dump(-1); }
-std::string num(double v) { return std::to_string(v); }
+// ERROR {"job":id,"message":...}
+std::string error_line(int64_t job, const std::string& message) {
+  return LineFields().add("job", job).add("message", message).line("ERROR");
+}
 
 // Lines from stdin, read on their own thread; an empty optional marks EOF.
 class LineQueue {
@@ -127,7 +130,7 @@ class Service {
       id = (int64_t)jid->as_number();
       start(id, req);
     } catch (const std::exception& e) {
-      emit("ERROR {\"job\":" + std::to_string(id) + ",\"message\":" + jstr(e.what()) + "}\n");
+      emit(error_line(id, e.what()));
     }
   }
 
@@ -179,7 +182,7 @@ class Service {
     if (s != RTG_OK) {
       const std::string msg = rtg_last_error(ctx_);
       for (Job* j : owner) {
-        emit("ERROR {\"job\":" + std::to_string(j->id) + ",\"message\":" + jstr(msg) + "}\n");
+        emit(error_line(j->id, msg));
         release(*j);
       }
       jobs_.remove_if([](const Job& j) { return j.acc == nullptr; });
@@ -225,44 +228,18 @@ class Service {
     if (!scene || !scene->is_object()) throw std::runtime_error("job without a \"scene\" object");
     SceneDesc desc = parse_scene(scene->dump(-1));
     RenderSettings& R = desc.render;
-    auto num_of = [&](const char* k, double& out) {
-      if (const Json* v = req.get(k)) {
-        if (!v->is_number()) throw SceneError(std::string(k) + ": expected a number");
-        out = v->as_number();
-        return true;
-      }
-      return false;
-    };
-    double v = 0;
-    if (num_of("width", v)) R.width = (int)v;
-    if (num_of("height", v)) R.height = (int)v;
-    if (num_of("spp", v)) R.spp = (int)v;
-    if (num_of("max_depth", v)) R.max_depth = (int)v;
-    if (num_of("seed", v)) R.seed = (uint64_t)v;
-    if (num_of("exposure", v)) R.exposure = v;
-    if (const Json* b = req.get("no_bvh"); b && b->is_bool() && b->as_bool()) R.use_bvh = false;
-    if (const Json* b = req.get("no_light_sampling"); b && b->is_bool() && b->as_bool()) R.light_sampling = false;
-    if (const Json* s = req.get("integrator")) {
-      const std::string n = s->as_string();
-      if (n == "path") R.integrator = Integrator::Path;
-      else if (n == "direct") R.integrator = Integrator::Direct;
-      else if (n == "normals") R.integrator = Integrator::Normals;
-      else throw SceneError("--integrator: expected path, direct or normals");
-    }
+    Options o;
+    read_job_fields(req, o);  // the rt flags' overrides, as JSON keys
+    o.apply(R);
     if (const auto errs = validate_scene(desc); !errs.empty()) throw SceneError(errs.front());
     Job j;
     j.id = id;
     j.settings = R;
-    if (const Json* p = req.get("precision")) {
-      if (p->as_string() == "f64") j.precision = RTG_PRECISION_F64;
-      else if (p->as_string() != "f32") throw std::runtime_error("precision: expected f32 or f64");
-    }
-    if (num_of("max_pass", v)) j.max_pass = (uint32_t)std::max(1.0, v);
-    if (const Json* b = req.get("preview"); b && b->is_bool()) j.preview = b->as_bool();
-    bool supersede = supersede_;
-    if (const Json* b = req.get("supersede"); b && b->is_bool()) supersede = b->as_bool();
-    if (supersede)
-      for (Job& o : jobs_) o.cancelled = true;
+    j.precision = o.precision;
+    if (o.max_pass) j.max_pass = (uint32_t)std::max(1, *o.max_pass);
+    j.preview = o.preview;
+    if (o.supersede_job.value_or(supersede_))
+      for (Job& other : jobs_) other.cancelled = true;
     // The resident scene: everything but the camera and the render settings.
     Json key = scene_to_json(desc);
     Json stripped = Json::object();
@@ -321,13 +298,19 @@ class Service {
                           img.rgb.data()),
               ctx_, "rtg_resolve");
     const std::vector<uint8_t> png = encode_png(img);
-    const std::string hdr = "FRAME {\"job\":" + std::to_string(j.id) + ",\"pass\":" + std::to_string(j.pass) +
-                            ",\"spp_done\":" + std::to_string(j.spp_done) + ",\"spp_target\":" +
-                            std::to_string(j.settings.spp) + ",\"elapsed_ms\":" + num(ms_between(j.start, Clock::now())) +
-                            ",\"rays\":" + std::to_string(j.rays + j.shadow_rays) + ",\"width\":" +
-                            std::to_string(j.settings.width) + ",\"height\":" + std::to_string(j.settings.height) +
-                            ",\"finished\":" + (finished ? "true" : "false") + ",\"cancelled\":" +
-                            (cancelled ? "true" : "false") + ",\"bytes\":" + std::to_string(png.size()) + "}\n";
+    const std::string hdr = LineFields()
+                                .add("job", j.id)
+                                .add("pass", j.pass)
+                                .add("spp_done", j.spp_done)
+                                .add("spp_target", j.settings.spp)
+                                .add("elapsed_ms", ms_between(j.start, Clock::now()))
+                                .add("rays", j.rays + j.shadow_rays)
+                                .add("width", j.settings.width)
+                                .add("height", j.settings.height)
+                                .add("finished", finished)
+                                .add("cancelled", cancelled)
+                                .add("bytes", (uint64_t)png.size())
+                                .line("FRAME");
     emit(hdr, &png);
   }
 
@@ -336,12 +319,20 @@ class Service {
     if (cancelled) frame(j, false, true);
     const double ms = ms_between(j.start, Clock::now());
     const double mrays = ms > 0 ? (double)(j.rays + j.shadow_rays) / 1e3 / ms : 0;
-    emit("DONE {\"job\":" + std::to_string(j.id) + ",\"cancelled\":" + (cancelled ? "true" : "false") +
-         ",\"spp_done\":" + std::to_string(j.spp_done) + ",\"elapsed_ms\":" + num(ms) + ",\"rays\":" +
-         std::to_string(j.rays) + ",\"shadow_rays\":" + std::to_string(j.shadow_rays) + ",\"mrays_per_sec\":" +
-         num(mrays) + ",\"grays_per_sec\":" + num(mrays / 1e3) + ",\"invalid_samples\":" + std::to_string(j.invalid) +
-         ",\"threads\":1,\"scene_cached\":" + (j.scene_cached ? "true" : "false") + ",\"scene_prep_ms\":" +
-         num(j.prep_ms) + "}\n");
+    emit(LineFields()
+             .add("job", j.id)
+             .add("cancelled", cancelled)
+             .add("spp_done", j.spp_done)
+             .add("elapsed_ms", ms)
+             .add("rays", j.rays)
+             .add("shadow_rays", j.shadow_rays)
+             .add("mrays_per_sec", mrays)
+             .add("grays_per_sec", mrays / 1e3)
+             .add("invalid_samples", j.invalid)
+             .add("threads", 1)
+             .add("scene_cached", j.scene_cached)
+             .add("scene_prep_ms", j.prep_ms)
+             .line("DONE"));
     if (!quiet_)
       std::cerr << "job " << j.id << (cancelled ? " cancelled" : " done") << " after " << (long)ms << " ms\n";
     release(j);
@@ -384,7 +375,7 @@ int serve_main(int device, bool supersede, uint32_t tick_paths, uint64_t cache_b
   try {
     svc = std::make_unique<Service>(device, supersede, tick_paths, cache_bytes ? cache_bytes : (8ull << 30), quiet);
   } catch (const std::exception& e) {
-    std::cout << "ERROR {\"job\":-1,\"message\":" << jstr(e.what()) << "}\n" << std::flush;
+    std::cout << error_line(-1, e.what()) << std::flush;
     return 2;
   }
   LineQueue queue;
@@ -405,7 +396,7 @@ int serve_main(int device, bool supersede, uint32_t tick_paths, uint64_t cache_b
     try {
       svc->tick();
     } catch (const std::exception& e) {
-      std::cout << "ERROR {\"job\":-1,\"message\":" << jstr(e.what()) << "}\n" << std::flush;
+      std::cout << error_line(-1, e.what()) << std::flush;
       reader.detach();
       return 2;
     }

@@ -4,6 +4,8 @@ oracle/Makefile). Run in the container that has the reference checkout:
 
     make -C oracle && python tests/golden/make_golden.py
 
+`--messages` rewrites only the scene-format fixtures (validation messages of
+bad_scenes.json, dumps of dump_scenes.json) in reference_meta.json.
 Each fixture is data only: inputs and the reference's outputs.
 """
 import ctypes as C
@@ -39,6 +41,33 @@ def ray_set(seed, n, extent, axis_every=50):
     r = np.zeros((n, 8))
     r[:, :3], r[:, 3:6], r[:, 6], r[:, 7] = o, d, 1e-6, np.inf
     return r
+
+
+def scene_format(L, meta):
+    """Validation messages and normalised dumps of the scene reader."""
+    bad = json.load(open(os.path.join(HERE, "bad_scenes.json")))
+    msgs = []
+    for text in bad:
+        buf = C.create_string_buffer(4096)
+        L.ref_validate(text.encode(), buf, len(buf))
+        msgs.append(buf.value.decode())
+    meta["bad_scene_messages"] = msgs
+    dumps = []
+    for text in json.load(open(os.path.join(HERE, "dump_scenes.json"))):
+        buf = C.create_string_buffer(1 << 20)
+        L.ref_dump_scene(text.encode(), buf, len(buf))
+        dumps.append(buf.value.decode())
+    meta["dump_scene_texts"] = dumps
+
+
+def messages_only():
+    assert orc.have_ref(), "oracle/_ref/librtref.so missing: run make -C oracle in the reference container"
+    path = os.path.join(HERE, "reference_meta.json")
+    meta = json.load(open(path))
+    scene_format(orc.ref(), meta)
+    with open(path, "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+    print("updated", path)
 
 
 def main():
@@ -186,14 +215,9 @@ def main():
         dumps[fn] = hashlib.sha256(buf.value).hexdigest()
     meta["dump_scene_sha256"] = dumps
 
-    # 8. Validation messages of the reference for malformed scenes.
-    bad = json.load(open(os.path.join(HERE, "bad_scenes.json")))
-    msgs = []
-    for text in bad:
-        buf = C.create_string_buffer(4096)
-        L.ref_validate(text.encode(), buf, len(buf))
-        msgs.append(buf.value.decode())
-    meta["bad_scene_messages"] = msgs
+    # 8. Validation messages of the reference for malformed scenes, and
+    # normalised dumps of scenes with escapes and extreme numbers.
+    scene_format(L, meta)
 
     np.savez_compressed(OUT, **out)
     with open(os.path.join(HERE, "reference_meta.json"), "w") as f:
@@ -202,4 +226,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    messages_only() if "--messages" in sys.argv[1:] else main()

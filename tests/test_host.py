@@ -84,6 +84,18 @@ def test_dump_scene_matches_reference():
         assert rtg.dump_scene(got) == got
 
 
+def test_dump_of_escapes_and_extreme_numbers_matches_reference():
+    """Unicode escapes (surrogate pairs), control characters, -0, 1e-300 and
+    the largest double survive the reader and print as the reference prints
+    them."""
+    cases = json.load(open(os.path.join(ROOT, "tests", "golden", "dump_scenes.json")))
+    assert len(cases) == len(META["dump_scene_texts"])
+    for text, want in zip(cases, META["dump_scene_texts"]):
+        got = rtg.dump_scene(text)
+        assert got == want
+        assert rtg.dump_scene(got) == got
+
+
 def test_validation_messages_match_reference():
     for text, want in zip(BAD, META["bad_scene_messages"]):
         assert rtg.validate(text) == want, text[:80]
