@@ -54,19 +54,40 @@ def render_once(rtg, ctx, dev, acc, cfg, spp, max_pass=32):
     return st.rays + st.shadow_rays
 
 
-def cpu_baseline(scene_text, width, height, spp_full, target_s=12.0):
+def host_cpus():
+    """The host cores this process may use: the affinity set, capped by
+    OMP_NUM_THREADS when it is set (the box's CPU share for one GPU), and
+    what the machine reports (nproc, model name)."""
+    affinity = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    threads = min(affinity, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else affinity
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": threads, "nproc": os.cpu_count(), "affinity": affinity, "omp_num_threads": cap,
+            "model": model}
+
+
+def cpu_baseline(scene_text, width, height, spp_full, target_s=12.0, single_s=4.0):
     """The reference renderer (oracle/_ref, built from the reference's own
     sources) on this host's cores, on a bounded sample of the workload:
     whole renders of the full image at a sample count of at most the
-    workload's (spp_full), chosen so that the sample takes ~target_s seconds."""
+    workload's (spp_full), chosen so that the sample takes ~target_s seconds
+    on every usable core; then ~single_s seconds on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
-    threads = max(1, min(16, os.cpu_count() or 1))
+    cpus = host_cpus()
     if orc.have_ref():
         kind = "reference"
 
-        def run(spp):
-            r = orc.RefScene(scene_text, width=width, height=height, spp=spp)
+        def run(spp, threads, w=width, h=height):
+            r = orc.RefScene(scene_text, width=w, height=h, spp=spp)
             stats = (C.c_uint64 * 5)()
             ms = C.c_double()
             orc.ref().ref_render_full(r.h, threads, 1, 32, None, stats, C.byref(ms))
@@ -74,20 +95,33 @@ def cpu_baseline(scene_text, width, height, spp_full, target_s=12.0):
     else:
         kind = "port"
 
-        def run(spp):
-            o = orc.OracleScene(json.loads(scene_text), width=width, height=height)
+        def run(spp, threads, w=width, h=height):
+            o = orc.OracleScene(json.loads(scene_text), width=w, height=h)
             t = time.perf_counter()
             _, _, st = o.render(0, spp, threads=threads)
             return st["rays"] + st["shadow_rays"], time.perf_counter() - t
-    rays1, s1 = run(1)
-    spp = int(max(1, min(spp_full, round(target_s / max(s1, 1e-3)))))
-    rays, secs, reps = 0, 0.0, 0
-    while secs < target_s * 0.8 and reps < 8:  # whole renders until ~10 s of CPU work
-        r, t = run(spp) if spp > 1 else (rays1, s1)
-        rays, secs, reps = rays + r, secs + t, reps + 1
+
+    def sample(threads, budget_s, w, h):
+        rays1, s1 = run(1, threads, w, h)
+        spp = int(max(1, min(spp_full, round(budget_s / max(s1, 1e-3)))))
+        rays, secs, reps = 0, 0.0, 0
+        while secs < budget_s * 0.8 and reps < 8:  # whole renders until ~budget_s of CPU work
+            r, t = run(spp, threads, w, h) if spp > 1 else (rays1, s1)
+            rays, secs, reps = rays + r, secs + t, reps + 1
+        return rays, secs, reps, spp
+
+    threads = cpus["threads"]
+    rays, secs, reps, spp = sample(threads, target_s, width, height)
+    # One thread: a quarter of the image (every fourth row band would change
+    # nothing per ray; a smaller image keeps the sample short).
+    w1, h1 = max(1, width // 2), max(1, height // 2)
+    rays_1, secs_1, reps_1, spp_1 = sample(1, single_s, w1, h1)
     return {"value": rays / secs / 1e9, "unit": "G rays/s", "cores": threads, "kind": kind,
             "sample": f"{reps} x {width}x{height} at {spp} spp (of {spp_full}), progressive passes, "
-                      f"{rays} rays in {secs:.2f} s"}
+                      f"{rays} rays in {secs:.2f} s on {threads} threads",
+            "host": cpus,
+            "single_thread": {"value": rays_1 / secs_1 / 1e9, "unit": "G rays/s", "cores": 1,
+                              "sample": f"{reps_1} x {w1}x{h1} at {spp_1} spp, {rays_1} rays in {secs_1:.2f} s"}}
 
 
 def extend_roofline(prof, traffic):
@@ -101,8 +135,12 @@ def extend_roofline(prof, traffic):
     per_launch = bytes_total / n_launch
     avg_ms = ext["ms"] / n_launch
     achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    # What the HBM itself carries per launch (FETCH_SIZE + WRITE_SIZE counters,
+    # scaled by a copy of known size) over the same launch time.
+    hbm = traffic / (avg_ms * 1e-3) / 1e9 if traffic and avg_ms > 0 else None
     return {"kernel": "k_extend (FP32 closest-hit traversal)", "bound": "hbm", "achieved": achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "hbm_achieved": hbm, "hbm_frac": hbm / HBM_PEAK_GBS if hbm is not None else None,
             "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches": ext["launches"],
             "node_visits_per_ray": prof["closest_visits"] / max(1, prof["closest_rays"]),
             "tests_per_ray": prof["closest_tests"] / max(1, prof["closest_rays"]),
@@ -274,6 +312,7 @@ def main():
     ap.add_argument("--profile-all", action="store_true",
                     help="time every kernel's launches in the timed region (default: the closest-hit kernel's)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
+    ap.add_argument("--no-f64", action="store_true", help="skip the FP64 (reference-precision) renders")
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
     ap.add_argument("--qps", type=float, default=6.0, help="served workload: request arrival rate")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="served workload: trace length")
@@ -350,7 +389,7 @@ def main():
         render_once(rtg, ctx, dev, acc, cfg, spp)
     ctx.synchronize()
     sprof = rtg.get_profile(ctx).as_dict()
-    roofline_serial = extend_roofline(sprof, traffic)
+    roofline_serial = extend_roofline(sprof, None)  # the traffic file holds the timed render's launches
     roofline_serial["kernel_ms_per_render"] = {k: round(v["ms"] / 2, 3) for k, v in sprof.items()
                                                if isinstance(v, dict) and v["launches"]}
     rtg.set_profiling(ctx, False)
@@ -359,6 +398,19 @@ def main():
             del os.environ[k]
         else:
             os.environ[k] = v
+    # The reference's own precision: the same renders in FP64 (double
+    # arithmetic, the reference's PCG32 stream; bit-identical hits).
+    f64 = None
+    if precision == rtg.PRECISION_F32 and not args.no_f64:
+        cfg64 = hs.settings(rtg.PRECISION_F64)
+        render_once(rtg, ctx, dev, acc, cfg64, spp)
+        ctx.synchronize()
+        t64 = time.perf_counter()
+        rays64 = sum(render_once(rtg, ctx, dev, acc, cfg64, spp) for _ in range(2))
+        ctx.synchronize()
+        s64 = time.perf_counter() - t64
+        f64 = {"value": rays64 / s64 / 1e9, "unit": "G rays/s", "ms_per_render": s64 / 2 * 1e3, "renders": 2,
+               "dtype": "f64"}
     dev.release()
     acc.release()
     ctx.close()
@@ -389,6 +441,7 @@ def main():
         "node_traffic": {"bytes_per_step": node_bytes // args.steps, "achieved": node_rate, "unit": "GB/s",
                          "peak": HBM_PEAK_GBS, "frac": node_rate / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
+        "f64": f64,
         "batched": batch,
         "served": served,
     }
