@@ -56,6 +56,7 @@ __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
 
 // Path-state slots of a context.
 constexpr int kBatchBuffers = 2;
+constexpr int kBatchBuffers2 = 5;  // batch groups of the second lane
 constexpr int kLaneBuffers = 3;    // FP32 chunks of the second lane
 constexpr int kLane64Buffers = 4;  // FP64 chunks of the second lane
 // Batches: every job's queue storage is rounded up to kSubQueues sub-queues
@@ -84,8 +85,8 @@ struct rtg_context {
   uint32_t job_stats_cap = 0;
   int n_cu = 0;
   // Path state, grown on demand: FP32 chunks, FP64 chunks, FP32 batches.
-  void* path_mem[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  uint64_t path_bytes[5] = {0, 0, 0, 0, 0};
+  void* path_mem[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  uint64_t path_bytes[6] = {0, 0, 0, 0, 0, 0};
   PathBuffers<float> pb32{};
   // A second lane for FP32 path-integrator passes: chunks alternate between
   // (stream, side, pb32) and (stream2, side2, pb32b), so two chunks' kernels
@@ -97,12 +98,17 @@ struct rtg_context {
   hipEvent_t fork = nullptr, join = nullptr;
   hipEvent_t acc_done[2] = {nullptr, nullptr};  // each lane's latest accumulation
   PathBuffers<double> pb64{};
-  PathBuffers<float> pbb{};
-  uint32_t cap[5] = {0, 0, 0, 0, 0};
-  // Batches: device job table and plans, pinned host staging of both.
-  BatchJob* batch_jobs = nullptr;
-  uint32_t* batch_plans = nullptr;  // job count, then 5 x (kMaxBatchJobs + 1): gen, acc, act, shd x 2
-  void* batch_host = nullptr;       // job table + job count + gen/acc plans, pinned
+  PathBuffers<float> pbb{}, pbb2{};  // batch groups, one set per lane
+  uint32_t cap[6] = {0, 0, 0, 0, 0, 0};
+  // Batches: device job table and plans per lane; pinned host staging of
+  // the job tables and plans of every group of a call, and of their
+  // statistics slots.
+  BatchJob* batch_jobs[2] = {nullptr, nullptr};
+  uint32_t* batch_plans[2] = {nullptr, nullptr};  // job count, then 5 x (kMaxBatchJobs + 1): gen, acc, act, shd x 2
+  void* batch_stage = nullptr;
+  size_t batch_stage_cap = 0;  // groups
+  unsigned long long* batch_stats_host = nullptr;
+  size_t batch_stats_cap = 0;  // segments
   uint32_t batch_occ[3][3] = {};    // per stack class: trace, shade, other workgroups per CU
   // Captured batch passes, one per (max depth, stack class): the launches of
   // a pass replay as one graph. Their arguments point into the batch
@@ -450,9 +456,12 @@ rtg_status rtg_close(rtg_context* ctx) {
   if (ctx->stats_host) hipHostFree(ctx->stats_host);
   if (ctx->job_stats) hipFree(ctx->job_stats);
   if (ctx->job_stats_host) hipHostFree(ctx->job_stats_host);
-  if (ctx->batch_jobs) hipFree(ctx->batch_jobs);
-  if (ctx->batch_plans) hipFree(ctx->batch_plans);
-  if (ctx->batch_host) hipHostFree(ctx->batch_host);
+  for (int l = 0; l < 2; ++l) {
+    if (ctx->batch_jobs[l]) hipFree(ctx->batch_jobs[l]);
+    if (ctx->batch_plans[l]) hipFree(ctx->batch_plans[l]);
+  }
+  if (ctx->batch_stage) hipHostFree(ctx->batch_stage);
+  if (ctx->batch_stats_host) hipHostFree(ctx->batch_stats_host);
   if (ctx->rgb_dev) hipFree(ctx->rgb_dev);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
@@ -836,8 +845,8 @@ uint64_t lay_out_paths(char* base, uint32_t cap, bool batch, unsigned long long*
 template <class T>
 rtg_status ensure_paths(rtg_context* ctx, int which, uint32_t cap, PathBuffers<T>& B) {
   if (ctx->cap[which] >= cap) return RTG_OK;
-  const bool batch = which == kBatchBuffers;
-  if (batch) drop_batch_graphs(ctx);
+  const bool batch = which == kBatchBuffers || which == kBatchBuffers2;
+  if (which == kBatchBuffers) drop_batch_graphs(ctx);  // graphs hold the first lane's buffers
   if (ctx->path_mem[which]) {
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream2));
@@ -1118,13 +1127,12 @@ struct Segment {
   uint32_t job, p0, n_pix, s0, k;
 };
 
-// A job of at least this many paths fills the GPU by itself: it gains
-// nothing from sharing launches and runs faster through the single-render
-// kernels, whose scene and parameters are kernel arguments (the batch
-// kernels read them per virtual block from the job table, which also costs
-// them registers). Measured: one 1024x1024, 64-spp torus render in 64 Mi
-// paths, 55 ms alone vs 65 ms as single-job batches.
-constexpr uint64_t kBatchSoloPaths = 1ull << 22;
+// A group of at least this many paths runs as a wide batch pass (grids over
+// every path, direct launches: the launches cost nothing next to the work,
+// and the kernels keep the single-render kernels' registers); smaller
+// groups run the persistent batch kernels, whose fixed grids replay from a
+// captured graph.
+constexpr uint64_t kWidePaths = 1ull << 22;
 
 bool batchable(const rtg_job& j) {
   return j.settings.precision == RTG_PRECISION_F32 && j.settings.integrator == RTG_INTEGRATOR_PATH &&
@@ -1138,27 +1146,65 @@ bool batch_graphs_enabled() {
   return !(e && e[0] == '0');
 }
 
-rtg_status ensure_batch_tables(rtg_context* ctx) {
-  if (ctx->batch_jobs) return RTG_OK;
-  HIP_TRY(ctx, hipMalloc(&ctx->batch_jobs, sizeof(BatchJob) * kMaxBatchJobs));
-  HIP_TRY(ctx, hipMalloc(&ctx->batch_plans, sizeof(uint32_t) * (1 + 5 * (kMaxBatchJobs + 1))));
-  HIP_TRY(ctx, hipHostMalloc(&ctx->batch_host,
-                             sizeof(BatchJob) * kMaxBatchJobs + sizeof(uint32_t) * (1 + 2 * (kMaxBatchJobs + 1))));
+constexpr size_t kPlanWords = 1 + 5 * (kMaxBatchJobs + 1);
+// Host staging of one group: its job table, then the job count and the
+// generate / accumulate plans.
+constexpr size_t kStageBytes = sizeof(BatchJob) * kMaxBatchJobs + sizeof(uint32_t) * (1 + 2 * (kMaxBatchJobs + 1));
+constexpr size_t kJobStatWords = (size_t)kBatchStatSlots * kStatStride;
+
+rtg_status ensure_batch_tables(rtg_context* ctx, size_t groups, size_t segments) {
+  for (int l = 0; l < 2; ++l)
+    if (!ctx->batch_jobs[l]) {
+      HIP_TRY(ctx, hipMalloc(&ctx->batch_jobs[l], sizeof(BatchJob) * kMaxBatchJobs));
+      HIP_TRY(ctx, hipMalloc(&ctx->batch_plans[l], sizeof(uint32_t) * kPlanWords));
+    }
+  // Previous calls have completed (each ends with a synchronisation), so
+  // the pinned buffers are free to be replaced.
+  if (ctx->batch_stage_cap < groups) {
+    if (ctx->batch_stage) HIP_TRY(ctx, hipHostFree(ctx->batch_stage));
+    ctx->batch_stage = nullptr;
+    ctx->batch_stage_cap = 0;
+    HIP_TRY(ctx, hipHostMalloc(&ctx->batch_stage, kStageBytes * groups));
+    ctx->batch_stage_cap = groups;
+  }
+  if (ctx->batch_stats_cap < segments) {
+    if (ctx->batch_stats_host) HIP_TRY(ctx, hipHostFree(ctx->batch_stats_host));
+    ctx->batch_stats_host = nullptr;
+    ctx->batch_stats_cap = 0;
+    HIP_TRY(ctx, hipHostMalloc(&ctx->batch_stats_host, sizeof(unsigned long long) * kJobStatWords * segments));
+    ctx->batch_stats_cap = segments;
+  }
   return RTG_OK;
 }
 
 int stack_class(uint32_t stack) { return stack_class_of(stack); }
 
-// One batched pass over segments seg[0..n) (n <= kMaxBatchJobs, total paths
-// <= the batch capacity). Adds each segment's counters to its job's stats.
-rtg_status run_group(rtg_context* ctx, const rtg_job* jobs, const Segment* seg, uint32_t n,
-                     std::vector<rtg_stats>& acc_stats) {
-  auto* hj = static_cast<BatchJob*>(ctx->batch_host);
+// One lane of batch groups: a main stream, its side stream, path buffers
+// and device job tables.
+struct BatchLane {
+  hipStream_t stream;
+  const SideStream* side;
+  const PathBuffers<float>* B;
+  BatchJob* jobs;
+  uint32_t* plans;
+  uint32_t cap;
+};
+
+// Enqueues one batched pass over segments seg[0..n) (n <= kMaxBatchJobs,
+// total paths <= the lane's capacity) on `lane`, staging its tables in
+// `stage` and copying its statistics slots to `stats_out` (pinned; read
+// after the call's final synchronisation). The accumulation waits for
+// `acc_wait` and records `acc_done` (if not null): the other lane's groups
+// may write the same pixels in an earlier sample range.
+rtg_status run_group(rtg_context* ctx, const rtg_job* jobs, const Segment* seg, uint32_t n, const BatchLane& lane,
+                     bool first_lane, void* stage, unsigned long long* stats_out, hipEvent_t acc_wait,
+                     hipEvent_t acc_done) {
+  auto* hj = static_cast<BatchJob*>(stage);
   auto* hn = reinterpret_cast<uint32_t*>(hj + kMaxBatchJobs);
   uint32_t* hgen = hn + 1;
   uint32_t* hacc = hgen + (kMaxBatchJobs + 1);
   hn[0] = n;
-  const PathBuffers<float>& B = ctx->pbb;
+  const PathBuffers<float>& B = *lane.B;
   uint64_t pid = 0, q = 0;
   uint32_t gen_blocks = 0, acc_blocks = 0, path_blocks = 0, stack = 1;
   int depth = 1;
@@ -1177,7 +1223,7 @@ rtg_status run_group(rtg_context* ctx, const rtg_job* jobs, const Segment* seg, 
     P.stat_slots = kBatchStatSlots;
     P.zero_stats = 1;
     P.counts = B.counts + (size_t)i * kBatchCounts;
-    P.stats = B.stats + (size_t)i * kBatchStatSlots * kStatStride;
+    P.stats = B.stats + (size_t)i * kJobStatWords;
     pid += P.n_paths;
     q += (uint64_t)P.sub_cap * kSubQueues;
     hj[i].S = J.scene->dev;
@@ -1192,66 +1238,65 @@ rtg_status run_group(rtg_context* ctx, const rtg_job* jobs, const Segment* seg, 
   }
   hgen[n] = gen_blocks;
   hacc[n] = acc_blocks;
-  if (pid > ctx->cap[kBatchBuffers] || q > (uint64_t)ctx->cap[kBatchBuffers] + kBatchQueueSlack)
+  if (pid > lane.cap || q > (uint64_t)lane.cap + kBatchQueueSlack)
     return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_render_batch: internal error: group exceeds the batch buffers");
-  uint32_t* plans = ctx->batch_plans;
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->batch_jobs, hj, sizeof(BatchJob) * n, hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(ctx, hipMemcpyAsync(plans, hn, sizeof(uint32_t) * (1 + 2 * (kMaxBatchJobs + 1)), hipMemcpyHostToDevice,
-                              ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(lane.jobs, hj, sizeof(BatchJob) * n, hipMemcpyHostToDevice, lane.stream));
+  HIP_TRY(ctx, hipMemcpyAsync(lane.plans, hn, sizeof(uint32_t) * (1 + 2 * (kMaxBatchJobs + 1)), hipMemcpyHostToDevice,
+                              lane.stream));
   constexpr uint32_t kP = kMaxBatchJobs + 1;
+  uint32_t* plans = lane.plans;
   const BatchPlans L{plans, plans + 1, plans + 1 + kP, plans + 1 + 2 * kP, plans + 1 + 3 * kP};
-  const int cls = stack_class(stack);
-  uint32_t* occ = ctx->batch_occ[cls];
-  if (!occ[0]) batch_occupancy(stack, &occ[0], &occ[1], &occ[2]);
-  const uint32_t cu = (uint32_t)std::max(1, ctx->n_cu);
-  // Persistent grids: resident workgroups walk the virtual blocks. A direct
-  // launch trims them to the work; a graph keeps them fixed (its launches
-  // replay with the arguments they were captured with) and the surplus
-  // workgroups find no block and exit.
-  const bool graph = !ctx->profiling && batch_graphs_enabled();
-  const SideStream* side = side_of(ctx);
-  BatchGrids g;
-  g.generate = std::max(1u, graph ? occ[2] * cu : std::min(gen_blocks, occ[2] * cu));
-  g.trace = std::max(1u, graph ? occ[0] * cu : std::min(path_blocks, occ[0] * cu));
-  g.shade = std::max(1u, graph ? occ[1] * cu : std::min(path_blocks, occ[1] * cu));
-  g.accumulate = std::max(1u, graph ? occ[2] * cu : std::min(acc_blocks, occ[2] * cu));
-  if (graph) {
-    hipGraphExec_t exec = nullptr;
-    for (const auto& e : ctx->batch_graphs)
-      if (e.depth == depth && e.cls == cls && e.side == (side != nullptr)) exec = e.exec;
-    if (!exec) {
-      hipGraph_t gr = nullptr;
-      HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      run_batch(ctx->batch_jobs, B, L, depth, stack, g, ctx->stream, side, nullptr);
-      const hipError_t le = hipGetLastError();
-      const hipError_t ce = hipStreamEndCapture(ctx->stream, &gr);
-      hipError_t ie = hipSuccess;
-      if (le == hipSuccess && ce == hipSuccess) ie = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
-      if (gr) hipGraphDestroy(gr);
-      for (hipError_t e : {le, ce, ie})
-        if (e != hipSuccess)
-          return fail(ctx, RTG_ERR_HIP, std::string("rtg_render_batch: capturing a batch pass: ") + hipGetErrorString(e));
-      ctx->batch_graphs.push_back({depth, cls, side != nullptr, exec});
-    }
-    HIP_TRY(ctx, hipGraphLaunch(exec, ctx->stream));
-  } else {
-    run_batch(ctx->batch_jobs, B, L, depth, stack, g, ctx->stream, side, hooks_of(ctx));
+  if (pid >= kWidePaths) {
+    BatchGrids g;
+    g.generate = std::max(1u, gen_blocks);
+    g.trace = g.shade = std::max(1u, path_blocks);
+    g.accumulate = std::max(1u, acc_blocks);
+    run_wide(lane.jobs, B, L, depth, stack, g, lane.stream, lane.side, hooks_of(ctx), acc_wait, acc_done);
     HIP_TRY(ctx, hipGetLastError());
+  } else {
+    const int cls = stack_class(stack);
+    uint32_t* occ = ctx->batch_occ[cls];
+    if (!occ[0]) batch_occupancy(stack, &occ[0], &occ[1], &occ[2]);
+    const uint32_t cu = (uint32_t)std::max(1, ctx->n_cu);
+    // Persistent grids: resident workgroups walk the virtual blocks. A direct
+    // launch trims them to the work; a graph keeps them fixed (its launches
+    // replay with the arguments they were captured with) and the surplus
+    // workgroups find no block and exit. Graphs hold the first lane's buffers.
+    const bool graph = first_lane && !ctx->profiling && batch_graphs_enabled();
+    BatchGrids g;
+    g.generate = std::max(1u, graph ? occ[2] * cu : std::min(gen_blocks, occ[2] * cu));
+    g.trace = std::max(1u, graph ? occ[0] * cu : std::min(path_blocks, occ[0] * cu));
+    g.shade = std::max(1u, graph ? occ[1] * cu : std::min(path_blocks, occ[1] * cu));
+    g.accumulate = std::max(1u, graph ? occ[2] * cu : std::min(acc_blocks, occ[2] * cu));
+    if (acc_wait) HIP_TRY(ctx, hipStreamWaitEvent(lane.stream, acc_wait, 0));
+    if (graph) {
+      hipGraphExec_t exec = nullptr;
+      for (const auto& e : ctx->batch_graphs)
+        if (e.depth == depth && e.cls == cls && e.side == (lane.side != nullptr)) exec = e.exec;
+      if (!exec) {
+        hipGraph_t gr = nullptr;
+        HIP_TRY(ctx, hipStreamBeginCapture(lane.stream, hipStreamCaptureModeThreadLocal));
+        run_batch(lane.jobs, B, L, depth, stack, g, lane.stream, lane.side, nullptr);
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipStreamEndCapture(lane.stream, &gr);
+        hipError_t ie = hipSuccess;
+        if (le == hipSuccess && ce == hipSuccess) ie = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+        if (gr) hipGraphDestroy(gr);
+        for (hipError_t e : {le, ce, ie})
+          if (e != hipSuccess)
+            return fail(ctx, RTG_ERR_HIP,
+                        std::string("rtg_render_batch: capturing a batch pass: ") + hipGetErrorString(e));
+        ctx->batch_graphs.push_back({depth, cls, lane.side != nullptr, exec});
+      }
+      HIP_TRY(ctx, hipGraphLaunch(exec, lane.stream));
+    } else {
+      run_batch(lane.jobs, B, L, depth, stack, g, lane.stream, lane.side, hooks_of(ctx));
+      HIP_TRY(ctx, hipGetLastError());
+    }
+    if (acc_done) HIP_TRY(ctx, hipEventRecord(acc_done, lane.stream));
   }
-  static thread_local std::vector<unsigned long long> slots;
-  slots.assign((size_t)n * kBatchStatSlots * kStatStride, 0);
-  HIP_TRY(ctx, hipMemcpyAsync(slots.data(), B.stats, slots.size() * sizeof(unsigned long long),
-                              hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  unsigned long long tot[kStCount] = {0};
-  for (uint32_t i = 0; i < n; ++i) {
-    unsigned long long h[kStCount] = {0};
-    for (uint32_t sl = 0; sl < kBatchStatSlots; ++sl)
-      for (int k = 0; k < kStCount; ++k) h[k] += slots[((size_t)i * kBatchStatSlots + sl) * kStatStride + k];
-    add_counters(&acc_stats[seg[i].job], h);
-    for (int k = 0; k < kStCount; ++k) tot[k] += h[k];
-  }
-  profile_resolve(ctx, tot);
+  HIP_TRY(ctx, hipMemcpyAsync(stats_out, B.stats, sizeof(unsigned long long) * kJobStatWords * n,
+                              hipMemcpyDeviceToHost, lane.stream));
   return RTG_OK;
 }
 
@@ -1380,26 +1425,26 @@ rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jo
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<rtg_stats> acc_stats(n_jobs, rtg_stats{});
-  // Jobs outside the batch kernels' scope, and jobs large enough to fill
-  // the GPU alone, through the single-render kernels: one after another with
-  // no host synchronisation (run_items, their chunks alternating between the
-  // lanes), each with its own statistics slots. Jobs write different
-  // accumulation buffers or disjoint pixels, so their order does not matter.
+  // Jobs outside the batch kernels' scope (FP64, direct / normals, deeper
+  // than kBatchDepth) through the single-render kernels: one after another
+  // (run_items, their chunks alternating between the lanes), each with its
+  // own statistics slots. Every FP32 path-integrator job, whatever its
+  // size, shares the batch launches. Jobs write different accumulation
+  // buffers or disjoint pixels, so their order does not matter.
   uint64_t batch_paths = 0;
   std::vector<uint8_t> in_batch(n_jobs, 0);
   std::vector<uint32_t> solo;
   for (uint32_t i = 0; i < n_jobs; ++i) {
-    const uint64_t paths = (uint64_t)(jobs[i].s1 - jobs[i].s0) * (p1[i] - p0[i]);
-    if (batchable(jobs[i]) && paths < kBatchSoloPaths) {
+    if (batchable(jobs[i])) {
       in_batch[i] = 1;
-      batch_paths += paths;
+      batch_paths += (uint64_t)(jobs[i].s1 - jobs[i].s0) * (p1[i] - p0[i]);
     } else {
       solo.push_back(i);
     }
   }
+  const size_t region = (size_t)kStatSlots * kStatStride;
   if (!solo.empty()) {
     const uint32_t ns = (uint32_t)solo.size();
-    const size_t region = (size_t)kStatSlots * kStatStride;
     if (ctx->job_stats_cap < ns) {
       if (ctx->job_stats) HIP_TRY(ctx, hipFree(ctx->job_stats));
       if (ctx->job_stats_host) HIP_TRY(ctx, hipHostFree(ctx->job_stats_host));
@@ -1419,24 +1464,14 @@ rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jo
     if (rtg_status st = run_items(ctx, items.data(), ns, nullptr, nullptr)) return st;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->job_stats_host, ctx->job_stats, sizeof(unsigned long long) * region * ns,
                                 hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    unsigned long long tot[kStCount] = {0};
-    for (uint32_t k = 0; k < ns; ++k) {
-      unsigned long long h[kStCount] = {0};
-      for (int sl = 0; sl < kStatSlots; ++sl)
-        for (int c = 0; c < kStCount; ++c) h[c] += ctx->job_stats_host[k * region + (size_t)sl * kStatStride + c];
-      add_counters(&acc_stats[solo[k]], h);
-      for (int c = 0; c < kStCount; ++c) tot[c] += h[c];
-    }
-    profile_resolve(ctx, tot);
   }
+  // Segments of at most `cap` paths (whole pixels x a sample sub-range),
+  // packed in job order into groups of at most `cap` paths and
+  // kMaxBatchJobs segments.
+  std::vector<Segment> segs;
+  std::vector<std::pair<size_t, size_t>> groups;
+  const uint32_t cap = batch_paths ? capacity_for(batch_paths) : 0;
   if (batch_paths) {
-    const uint32_t cap = capacity_for(batch_paths);
-    if (rtg_status st = ensure_paths(ctx, kBatchBuffers, cap, ctx->pbb)) return st;
-    if (rtg_status st = ensure_batch_tables(ctx)) return st;
-    // Segments of at most `cap` paths (whole pixels x a sample sub-range),
-    // packed in job order into groups of at most `cap` paths.
-    std::vector<Segment> segs;
     for (uint32_t i = 0; i < n_jobs; ++i) {
       if (!in_batch[i]) continue;
       const uint32_t k_total = jobs[i].s1 - jobs[i].s0;
@@ -1447,19 +1482,68 @@ rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jo
         sa += k;
       }
     }
-    size_t a = 0;
-    while (a < segs.size()) {
+    for (size_t a = 0; a < segs.size();) {
       size_t b = a;
       uint64_t paths = 0;
-      while (b < segs.size() && b - a < (size_t)kMaxBatchJobs &&
-             paths + (uint64_t)segs[b].n_pix * segs[b].k <= cap) {
+      while (b < segs.size() && b - a < (size_t)kMaxBatchJobs && paths + (uint64_t)segs[b].n_pix * segs[b].k <= cap) {
         paths += (uint64_t)segs[b].n_pix * segs[b].k;
         ++b;
       }
-      if (rtg_status st = run_group(ctx, jobs, segs.data() + a, (uint32_t)(b - a), acc_stats)) return st;
+      groups.emplace_back(a, b);
       a = b;
     }
+    // Groups alternate between two lanes (streams and path buffers), so one
+    // group's kernels fill the other's tails; a group's accumulation waits
+    // for the other lane's latest, so every pixel adds its sample ranges in
+    // order.
+    const bool two = groups.size() > 1 && two_lanes_enabled();
+    if (rtg_status st = ensure_paths(ctx, kBatchBuffers, cap, ctx->pbb)) return st;
+    if (two)
+      if (rtg_status st = ensure_paths(ctx, kBatchBuffers2, cap, ctx->pbb2)) return st;
+    if (rtg_status st = ensure_batch_tables(ctx, groups.size(), segs.size())) return st;
+    const BatchLane lanes[2] = {
+        {ctx->stream, side_of(ctx), &ctx->pbb, ctx->batch_jobs[0], ctx->batch_plans[0], ctx->cap[kBatchBuffers]},
+        {ctx->stream2, side_of(ctx, true), &ctx->pbb2, ctx->batch_jobs[1], ctx->batch_plans[1],
+         ctx->cap[kBatchBuffers2]}};
+    if (two) {
+      HIP_TRY(ctx, hipEventRecord(ctx->fork, ctx->stream));
+      HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream2, ctx->fork, 0));
+    }
+    bool used[2] = {false, false};
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      const int l = two ? (int)(gi & 1) : 0;
+      const hipEvent_t wait = (two && used[l ^ 1]) ? ctx->acc_done[l ^ 1] : nullptr;
+      const hipEvent_t rec = two ? ctx->acc_done[l] : nullptr;
+      char* stage = static_cast<char*>(ctx->batch_stage) + gi * kStageBytes;
+      unsigned long long* out = ctx->batch_stats_host + groups[gi].first * kJobStatWords;
+      if (rtg_status st = run_group(ctx, jobs, segs.data() + groups[gi].first,
+                                    (uint32_t)(groups[gi].second - groups[gi].first), lanes[l], l == 0, stage, out,
+                                    wait, rec))
+        return st;
+      used[l] = true;
+    }
+    if (used[1]) {
+      HIP_TRY(ctx, hipEventRecord(ctx->join, ctx->stream2));
+      HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->join, 0));
+    }
   }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  unsigned long long tot[kStCount] = {0};
+  for (size_t k = 0; k < solo.size(); ++k) {
+    unsigned long long h[kStCount] = {0};
+    for (int sl = 0; sl < kStatSlots; ++sl)
+      for (int c = 0; c < kStCount; ++c) h[c] += ctx->job_stats_host[k * region + (size_t)sl * kStatStride + c];
+    add_counters(&acc_stats[solo[k]], h);
+    for (int c = 0; c < kStCount; ++c) tot[c] += h[c];
+  }
+  for (size_t k = 0; k < segs.size(); ++k) {
+    unsigned long long h[kStCount] = {0};
+    for (uint32_t sl = 0; sl < kBatchStatSlots; ++sl)
+      for (int c = 0; c < kStCount; ++c) h[c] += ctx->batch_stats_host[(k * kBatchStatSlots + sl) * kStatStride + c];
+    add_counters(&acc_stats[segs[k].job], h);
+    for (int c = 0; c < kStCount; ++c) tot[c] += h[c];
+  }
+  profile_resolve(ctx, tot);
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (stats)
     for (uint32_t i = 0; i < n_jobs; ++i) {

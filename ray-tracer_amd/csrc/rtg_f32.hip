@@ -82,6 +82,38 @@ void batch_impl(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPl
   RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_batch_accumulate<float>, dim3(g.accumulate), dim3(kBlock), 0, s, jobs, B, L);
 }
 
+// A wide batch pass: the grids cover every path of the batch (path_blocks
+// virtual blocks; generate and accumulate blocks as planned on the host).
+template <int STACK>
+void wide_impl(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPlans& L, int depth, const BatchGrids& g,
+               hipStream_t s, const SideStream* side, const LaunchHooks* h, hipEvent_t acc_wait, hipEvent_t acc_done) {
+  RTG_LAUNCH(h, RTG_K_GENERATE, k_wide_generate<float>, dim3(g.generate), dim3(kBlock), 0, s, jobs, B, L);
+  RTG_LAUNCH(h, RTG_K_PLAN, k_batch_plan<float>, dim3(1), dim3(kMaxBatchJobs), 0, s, jobs, L, -1);
+  bool pending[2] = {false, false};
+  for (int b = 0; b < depth; ++b) {
+    RTG_LAUNCH(h, RTG_K_EXTEND, (k_wide_extend<float, STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
+    if (pending[b & 1]) hipStreamWaitEvent(s, side->shadowed[b & 1], 0);
+    pending[b & 1] = false;
+    RTG_LAUNCH(h, RTG_K_SHADE, k_wide_shade<float>, dim3(g.shade), dim3(kBlock), 0, s, jobs, B, L, b);
+    RTG_LAUNCH(h, RTG_K_PLAN, k_batch_plan<float>, dim3(1), dim3(kMaxBatchJobs), 0, s, jobs, L, b);
+    if (side) {
+      hipEventRecord(side->shaded, s);
+      hipStreamWaitEvent(side->stream, side->shaded, 0);
+      RTG_LAUNCH(h, RTG_K_SHADOW, (k_wide_shadow<float, STACK>), dim3(g.trace), dim3(kBlock), 0, side->stream, jobs,
+                 B, L, b);
+      hipEventRecord(side->shadowed[b & 1], side->stream);
+      pending[b & 1] = true;
+    } else {
+      RTG_LAUNCH(h, RTG_K_SHADOW, (k_wide_shadow<float, STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
+    }
+  }
+  for (int p = 0; p < 2; ++p)
+    if (pending[p]) hipStreamWaitEvent(s, side->shadowed[p], 0);
+  if (acc_wait) hipStreamWaitEvent(s, acc_wait, 0);
+  RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_wide_accumulate<float>, dim3(g.accumulate), dim3(kBlock), 0, s, jobs, B, L);
+  if (acc_done) hipEventRecord(acc_done, s);
+}
+
 template <int STACK>
 uint32_t occupancy_trace() {
   int n = 0;
@@ -114,6 +146,16 @@ void run_batch(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPla
     case 0: batch_impl<19>(jobs, B, L, depth, g, s, side, h); break;
     case 1: batch_impl<31>(jobs, B, L, depth, g, s, side, h); break;
     default: batch_impl<63>(jobs, B, L, depth, g, s, side, h); break;
+  }
+}
+
+void run_wide(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPlans& L, int depth, uint32_t stack,
+              const BatchGrids& g, hipStream_t s, const SideStream* side, const LaunchHooks* h, hipEvent_t acc_wait,
+              hipEvent_t acc_done) {
+  switch (stack_class_of(stack)) {
+    case 0: wide_impl<19>(jobs, B, L, depth, g, s, side, h, acc_wait, acc_done); break;
+    case 1: wide_impl<31>(jobs, B, L, depth, g, s, side, h, acc_wait, acc_done); break;
+    default: wide_impl<63>(jobs, B, L, depth, g, s, side, h, acc_wait, acc_done); break;
   }
 }
 

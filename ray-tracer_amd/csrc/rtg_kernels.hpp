@@ -951,4 +951,58 @@ __global__ __launch_bounds__(kBlock) void k_batch_accumulate(const BatchJob* __r
   batch_loop(L.acc, jobs_of(L), [&](uint32_t j, uint32_t lb) { accumulate_body<T>(B, jobs[j].P, lb, tile); });
 }
 
+// Wide batches: one virtual block per workgroup over a grid that covers
+// every path of the batch (workgroups past the plan's end exit at once), for
+// batches large enough that launching such grids costs nothing next to the
+// work. Without the persistent loop a workgroup holds no state across
+// blocks, so these kernels keep the registers and occupancy of the
+// single-render kernels.
+template <class F>
+__device__ __forceinline__ void batch_block(const uint32_t* __restrict__ plan, uint32_t n_jobs, F&& f) {
+  const uint32_t vb = blockIdx.x;
+  if (vb >= __builtin_amdgcn_readfirstlane(plan[n_jobs])) return;
+  const uint32_t j = job_of(plan, n_jobs, vb);
+  f(j, vb - __builtin_amdgcn_readfirstlane(plan[j]));
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_wide_generate(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
+                                                          BatchPlans L) {
+  batch_block(L.gen, jobs_of(L), [&](uint32_t j, uint32_t lb) { generate_body<T>(jobs[j].S, B, jobs[j].P, lb); });
+}
+
+template <class T, int STACK>
+__global__ __launch_bounds__(kBlock) void k_wide_extend(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
+                                                        BatchPlans L, int b) {
+  __shared__ uint32_t sref[STACK * kBlock];
+  __shared__ uint32_t pre[kSubQueues + 1];
+  batch_block(L.act, jobs_of(L), [&](uint32_t j, uint32_t lb) {
+    extend_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, pre);
+  });
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_wide_shade(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
+                                                       BatchPlans L, int b) {
+  __shared__ uint32_t pre[kSubQueues + 1];
+  batch_block(L.act, jobs_of(L), [&](uint32_t j, uint32_t lb) { shade_body<T>(jobs[j].S, B, jobs[j].P, b, lb, pre); });
+}
+
+template <class T, int STACK>
+__global__ __launch_bounds__(kBlock) void k_wide_shadow(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
+                                                        BatchPlans L, int b) {
+  __shared__ uint32_t sref[STACK * kBlock];
+  __shared__ uint32_t pre[kSubQueues + 1];
+  batch_block(L.shd + (b & 1) * (kMaxBatchJobs + 1), jobs_of(L), [&](uint32_t j, uint32_t lb) {
+    shadow_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, pre);
+  });
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_wide_accumulate(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
+                                                            BatchPlans L) {
+  __shared__ T tile[kAccLds<T>];
+  batch_block(L.acc, jobs_of(L), [&](uint32_t j, uint32_t lb) { accumulate_body<T>(B, jobs[j].P, lb, tile); });
+}
+
 }  // namespace rtg

@@ -71,6 +71,12 @@ struct BatchGrids {
 // shade, plan, shadow), accumulate over the jobs of `jobs` (device table).
 void run_batch(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPlans& L, int depth, uint32_t stack,
                const BatchGrids& g, hipStream_t s, const SideStream* side, const LaunchHooks* hooks);
+// One wide batch pass (grids covering every path of the batch, one virtual
+// block per workgroup); the accumulation waits for `acc_wait` and records
+// `acc_done` when not null.
+void run_wide(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPlans& L, int depth, uint32_t stack,
+              const BatchGrids& g, hipStream_t s, const SideStream* side, const LaunchHooks* hooks,
+              hipEvent_t acc_wait = nullptr, hipEvent_t acc_done = nullptr);
 // Resident workgroups per CU of the batch kernels (for the persistent grids).
 void batch_occupancy(uint32_t stack, uint32_t* trace, uint32_t* shade, uint32_t* other);
 

@@ -102,8 +102,9 @@ def test_pixel_ranges_of_one_buffer(ctx):
 
 
 def test_jobs_outside_the_batch_kernels(ctx):
-    """FP64 and direct/normals jobs, and jobs of at least 4 Mi paths, go
-    through the single-render path."""
+    """FP64 and direct/normals jobs go through the single-render path; a job
+    of 4 Mi paths shares a wide batch pass (grids over every path) with the
+    small one."""
     sp = [job_spec(scenes.load_asset("materials.json"), 20, 12, 2, precision=rtg.PRECISION_F64),
           job_spec(scenes.load_asset("materials.json"), 20, 12, 2, integrator="direct"),
           job_spec(scenes.load_asset("mirrors.json"), 16, 8, 2, integrator="normals"),
@@ -131,6 +132,21 @@ def test_many_small_jobs_in_several_groups(ctx, specs, monkeypatch):
         s1, c1, st1 = solo(ctx, o, cfg, 0, spp)
         assert np.array_equal(res[i][1], c1) and np.array_equal(res[i][0], s1), i
         assert stats[i].rays == st1.rays and stats[i].node_visits == st1.node_visits
+
+
+def test_sample_ranges_of_one_pixel_across_lanes(ctx, monkeypatch):
+    """Chunks of 1024 paths and 2500 samples per pixel: each pixel's samples
+    are cut into sub-ranges that land in groups on both lanes; the pixel's
+    additions must still come in sample order (bitwise the solo render)."""
+    monkeypatch.setenv("RTG_CHUNK_PATHS", "1024")
+    sp = [job_spec(scenes.load("cornell_c1.json"), 3, 1, 2500), job_spec(scenes.load_asset("first_light.json"), 2, 2, 700)]
+    res, (stats,) = batched(ctx, sp, [(0, 2500)])
+    for (o, cfg, spp), (s, c), st in zip(sp, res, stats):
+        s1, c1, st1 = solo(ctx, o, cfg, 0, spp)
+        assert int(c.min()) == spp
+        assert np.array_equal(c, c1) and np.array_equal(s.view(np.uint64), s1.view(np.uint64))
+        for k in COUNTERS:
+            assert getattr(st, k) == getattr(st1, k), k
 
 
 def test_graph_replays_equal_direct_launches(ctx, specs, monkeypatch):
