@@ -220,6 +220,23 @@ def batch_leg(rtg, seed=4, count=64):
     ctx.synchronize()
     batched_s = time.perf_counter() - t0
     identical = digests() == ref
+    # The autotuner picks the scheduling knobs (queue / chunk capacity, wide
+    # batch passes, lanes) by timing this workload's calls, then the batched
+    # workload is timed again under its choice.
+    pass_calls = [[(devs[i], accs[i], cfgs[i]) + scheds[i][p] for i in range(count) if p < len(scheds[i])]
+                  for p in range(max(len(s) for s in scheds))]
+    t0 = time.perf_counter()
+    tune = ctx.autotune(pass_calls, key=f"batch-seed{seed}-{count}")
+    tune["tuning_s"] = round(time.perf_counter() - t0, 2)
+    for a in accs:
+        a.clear()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for call in pass_calls:
+        ctx.render_batch(call)
+    ctx.synchronize()
+    tuned_s = time.perf_counter() - t0
+    identical = identical and digests() == ref
     for x in accs + devs:
         x.release()
     ctx.close()
@@ -236,6 +253,9 @@ def batch_leg(rtg, seed=4, count=64):
             "plain_g_rays_per_s": rays_p / plain_s / 1e9, "plain_passes_call_g_rays_per_s": rays_q / passes_s / 1e9,
             "batched_g_rays_per_s": rays_b / batched_s / 1e9,
             "speedup": plain_s / batched_s, "speedup_vs_passes_call": passes_s / batched_s,
+            "autotune": tune, "batched_tuned_s": round(tuned_s, 4),
+            "batched_tuned_g_rays_per_s": rays_b / tuned_s / 1e9,
+            "speedup_tuned": plain_s / tuned_s, "speedup_tuned_vs_passes_call": passes_s / tuned_s,
             "identical": identical and same_q and rays_p == rays_b == rays_q}
 
 

@@ -277,6 +277,42 @@ typedef struct rtg_job {
 rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jobs, rtg_stats* stats);
 
 /* ------------------------------------------------------------------------ */
+/* Scheduling knobs of a context. None of them changes a result (images and
+ * counters are bitwise independent of how passes are cut into chunks and
+ * groups and spread over lanes); they only change speed. Zero fields mean
+ * the built-in default. The reference has no counterpart: its tile size and
+ * thread count (RenderSettings, core/scene_desc.hpp:31-32) play this role
+ * for the CPU. */
+typedef struct rtg_tuning {
+  uint32_t chunk_paths; /* paths per chunk and per batch group (queue capacity), 1024..2^28; default 32 Mi */
+  uint32_t wide_paths;  /* batch groups of at least this many paths run as wide passes; default 4 Mi */
+  uint32_t lanes;       /* 1 or 2 lanes for chunks and batch groups; default 2 */
+  uint32_t pad;
+} rtg_tuning;
+/* The values in effect (defaults filled in). */
+rtg_status rtg_get_tuning(rtg_context* ctx, rtg_tuning* out);
+rtg_status rtg_set_tuning(rtg_context* ctx, const rtg_tuning* tuning);
+
+typedef struct rtg_tune_result {
+  rtg_tuning best;
+  double default_ms; /* the workload with the built-in defaults */
+  double best_ms;    /* the workload with `best` */
+  uint32_t trials;   /* configurations timed; 0 = taken from the cache */
+  uint32_t pad;
+} rtg_tune_result;
+/* The autotuner: times a workload — n_calls rtg_render_batch calls, call i
+ * rendering the next call_jobs[i] entries of `jobs` — under candidate
+ * tunings (coordinate descent over chunk_paths, wide_paths and lanes,
+ * starting from the defaults), keeps the fastest in the context and
+ * returns it with both timings. The jobs' accumulation buffers are
+ * overwritten and left cleared. With cache_path, results are cached per
+ * device (name, compute units) and `key` (the caller's name for the
+ * workload class; no quotes) in a JSON-lines file: a cached entry is used
+ * without timing, a new result is appended. */
+rtg_status rtg_autotune(rtg_context* ctx, const rtg_job* jobs, const uint32_t* call_jobs, uint32_t n_calls,
+                        const char* cache_path, const char* key, rtg_tune_result* out);
+
+/* ------------------------------------------------------------------------ */
 /* Batch ray queries: Scene::intersect / occluded / intersect_brute
  * (src/core/scene.cpp:176-234) over n rays.
  * Ray record: 8 doubles {ox, oy, oz, dx, dy, dz, tmin, tmax}.

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -130,6 +131,7 @@ struct rtg_context {
   rtg_profile profile{};
   LaunchHooks hooks{};
   char* scratch_dev = nullptr;  // this device's copy of `scratch` (rtg_render_pass_static)
+  rtg_tuning tuning{};          // scheduling knobs (zero: built-in default)
 };
 
 void drop_batch_graphs(rtg_context* ctx) {
@@ -864,15 +866,17 @@ rtg_status ensure_paths(rtg_context* ctx, int which, uint32_t cap, PathBuffers<T
   return RTG_OK;
 }
 
-// Paths per device chunk. Large chunks keep the later bounces' queues long
-// enough to fill 256 CUs (measured on the torus workload: 1 Mi paths 1.56,
-// 4 Mi 2.56, 16 Mi 2.99 G rays/s); 32 Mi paths of FP32 state are ~4.6 GB.
-uint32_t chunk_paths_default() {
+// Paths per device chunk and per batch group. Large chunks keep the later
+// bounces' queues long enough to fill 256 CUs (measured on the torus
+// workload: 1 Mi paths 1.56, 4 Mi 2.56, 16 Mi 2.99 G rays/s); 32 Mi paths of
+// FP32 state are ~4.6 GB. RTG_CHUNK_PATHS overrides the context's tuning.
+constexpr uint32_t kDefaultChunkPaths = 1u << 25;
+uint32_t chunk_paths_of(const rtg_context* ctx) {
   if (const char* e = std::getenv("RTG_CHUNK_PATHS")) {
     const long v = std::atol(e);
     if (v >= 1024 && v <= (1L << 28)) return (uint32_t)v;
   }
-  return 1u << 25;
+  return ctx->tuning.chunk_paths ? ctx->tuning.chunk_paths : kDefaultChunkPaths;
 }
 
 int32_t pid_layout_default() {
@@ -945,8 +949,8 @@ ChunkParams render_params(const rtg_render_settings* cfg, const rtg_accum* acc) 
 
 // Chunk capacity: the configured maximum, or less when the work is smaller
 // (rounded up to a power of two so buffers are reused across passes).
-uint32_t capacity_for(uint64_t paths) {
-  const uint32_t lim = chunk_paths_default();
+uint32_t capacity_for(const rtg_context* ctx, uint64_t paths) {
+  const uint32_t lim = chunk_paths_of(ctx);
   uint32_t cap = 1024;
   while (cap < paths && cap < lim) cap <<= 1;
   return std::min(cap, lim);
@@ -965,12 +969,14 @@ uint32_t static_capacity(bool f64, uint64_t bytes) {
 
 // Chunks of a path-integrator pass on two lanes (passes of at least
 // kTwoLanePaths paths, or of one full chunk if chunks are smaller) unless
-// RTG_LANES=1. The direct and normals kernels, one launch per chunk, fill
-// the GPU alone and lose ~7 % to the split (measured).
+// RTG_LANES=1 or the context is tuned to one lane. The direct and normals
+// kernels, one launch per chunk, fill the GPU alone and lose ~7 % to the
+// split (measured).
 constexpr uint64_t kTwoLanePaths = 1ull << 22;
-bool two_lanes_enabled() {
+bool two_lanes_enabled(const rtg_context* ctx) {
   const char* e = std::getenv("RTG_LANES");
-  return !(e && e[0] == '1');
+  if (e && e[0] == '1') return false;
+  return ctx->tuning.lanes != 1;
 }
 
 // One pass (samples [s0, s1) of pixels [px0, px1)) of one render, and the
@@ -997,11 +1003,10 @@ struct PassItem {
 // read. Returns the items completed in *done_out.
 rtg_status run_items(rtg_context* ctx, const PassItem* items, uint32_t n, const volatile int32_t* cancel,
                      uint32_t* done_out) {
-  const uint64_t chunk = chunk_paths_default();
+  const uint64_t chunk = chunk_paths_of(ctx);
   const uint64_t split_last = std::min<uint64_t>(kTwoLanePaths, chunk);
-  auto lanes_of = [&](uint32_t i) {
-    return items[i].cfg->integrator == RTG_INTEGRATOR_PATH && two_lanes_enabled();
-  };
+  const bool lanes = two_lanes_enabled(ctx);
+  auto lanes_of = [&](uint32_t i) { return items[i].cfg->integrator == RTG_INTEGRATOR_PATH && lanes; };
   auto paths_of = [&](uint32_t i) { return (uint64_t)(items[i].s1 - items[i].s0) * (items[i].px1 - items[i].px0); };
   // An item of at least `split` paths is cut into (at least) two chunks, one
   // per lane. An item followed by another shares the GPU with it anyway, so
@@ -1010,7 +1015,7 @@ rtg_status run_items(rtg_context* ctx, const PassItem* items, uint32_t n, const 
   // rays/s).
   auto cap_of = [&](uint32_t i) {
     const uint64_t paths = paths_of(i);
-    return capacity_for(lanes_of(i) && paths >= (i + 1 < n ? chunk : split_last) ? (paths + 1) / 2 : paths);
+    return capacity_for(ctx, lanes_of(i) && paths >= (i + 1 < n ? chunk : split_last) ? (paths + 1) / 2 : paths);
   };
   uint32_t cap32 = 0, cap64 = 0;
   bool two = false;
@@ -1132,7 +1137,10 @@ struct Segment {
 // and the kernels keep the single-render kernels' registers); smaller
 // groups run the persistent batch kernels, whose fixed grids replay from a
 // captured graph.
-constexpr uint64_t kWidePaths = 1ull << 22;
+constexpr uint64_t kDefaultWidePaths = 1ull << 22;
+uint64_t wide_paths_of(const rtg_context* ctx) {
+  return ctx->tuning.wide_paths ? ctx->tuning.wide_paths : kDefaultWidePaths;
+}
 
 bool batchable(const rtg_job& j) {
   return j.settings.precision == RTG_PRECISION_F32 && j.settings.integrator == RTG_INTEGRATOR_PATH &&
@@ -1246,7 +1254,7 @@ rtg_status run_group(rtg_context* ctx, const rtg_job* jobs, const Segment* seg, 
   constexpr uint32_t kP = kMaxBatchJobs + 1;
   uint32_t* plans = lane.plans;
   const BatchPlans L{plans, plans + 1, plans + 1 + kP, plans + 1 + 2 * kP, plans + 1 + 3 * kP};
-  if (pid >= kWidePaths) {
+  if (pid >= wide_paths_of(ctx)) {
     BatchGrids g;
     g.generate = std::max(1u, gen_blocks);
     g.trace = g.shade = std::max(1u, path_blocks);
@@ -1344,7 +1352,7 @@ rtg_status rtg_render_pass_static(rtg_context* ctx, const rtg_scene* scene, cons
   const uint64_t slot_bytes = ((uint64_t)n_slots * 8 + 255) & ~uint64_t(255);
   unsigned long long* slots = reinterpret_cast<unsigned long long*>(ctx->scratch_dev);
   char* base = ctx->scratch_dev + slot_bytes;
-  const uint32_t cap = std::min(capacity_for(paths), static_capacity(f64, kStaticScratchBytes - slot_bytes));
+  const uint32_t cap = std::min(capacity_for(ctx, paths), static_capacity(f64, kStaticScratchBytes - slot_bytes));
   if (cap == 0) return fail(ctx, RTG_ERR_UNSUPPORTED, fn() + ": the scratch array holds no chunk");
   PathBuffers<float> B32{};
   PathBuffers<double> B64{};
@@ -1470,7 +1478,7 @@ rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jo
   // kMaxBatchJobs segments.
   std::vector<Segment> segs;
   std::vector<std::pair<size_t, size_t>> groups;
-  const uint32_t cap = batch_paths ? capacity_for(batch_paths) : 0;
+  const uint32_t cap = batch_paths ? capacity_for(ctx, batch_paths) : 0;
   if (batch_paths) {
     for (uint32_t i = 0; i < n_jobs; ++i) {
       if (!in_batch[i]) continue;
@@ -1496,7 +1504,7 @@ rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jo
     // group's kernels fill the other's tails; a group's accumulation waits
     // for the other lane's latest, so every pixel adds its sample ranges in
     // order.
-    const bool two = groups.size() > 1 && two_lanes_enabled();
+    const bool two = groups.size() > 1 && two_lanes_enabled(ctx);
     if (rtg_status st = ensure_paths(ctx, kBatchBuffers, cap, ctx->pbb)) return st;
     if (two)
       if (rtg_status st = ensure_paths(ctx, kBatchBuffers2, cap, ctx->pbb2)) return st;
@@ -1550,6 +1558,174 @@ rtg_status rtg_render_batch(rtg_context* ctx, const rtg_job* jobs, uint32_t n_jo
       stats[i] = acc_stats[i];
       stats[i].elapsed_ms = ms;
     }
+  return RTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Scheduling knobs and the autotuner.
+rtg_status rtg_get_tuning(rtg_context* ctx, rtg_tuning* out) {
+  if (!ctx || !out) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_get_tuning: NULL argument");
+  *out = rtg_tuning{};
+  out->chunk_paths = chunk_paths_of(ctx);
+  out->wide_paths = (uint32_t)std::min<uint64_t>(wide_paths_of(ctx), 0xFFFFFFFFu);
+  out->lanes = two_lanes_enabled(ctx) ? 2 : 1;
+  return RTG_OK;
+}
+
+rtg_status rtg_set_tuning(rtg_context* ctx, const rtg_tuning* t) {
+  if (!ctx || !t) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_set_tuning: NULL argument");
+  if (t->chunk_paths && (t->chunk_paths < 1024 || t->chunk_paths > (1u << 28)))
+    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_set_tuning: chunk_paths must be in [1024, 2^28]");
+  if (t->lanes > 2) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_set_tuning: lanes must be 1 or 2");
+  ctx->tuning = *t;
+  ctx->tuning.pad = 0;
+  return RTG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// One cache line: {"device":"<name>","cus":N,"key":"<key>","chunk_paths":N,
+// "wide_paths":N,"lanes":N,"default_ms":X,"best_ms":X}.
+std::string tune_line(const std::string& dev, int cus, const std::string& key, const rtg_tune_result& r) {
+  char buf[512];
+  std::snprintf(buf, sizeof buf,
+                "{\"device\":\"%s\",\"cus\":%d,\"key\":\"%s\",\"chunk_paths\":%u,\"wide_paths\":%u,\"lanes\":%u,"
+                "\"default_ms\":%.3f,\"best_ms\":%.3f}",
+                dev.c_str(), cus, key.c_str(), r.best.chunk_paths, r.best.wide_paths, r.best.lanes, r.default_ms,
+                r.best_ms);
+  return buf;
+}
+
+bool tune_field(const std::string& line, const char* name, double* v) {
+  const std::string tag = std::string("\"") + name + "\":";
+  const size_t at = line.find(tag);
+  if (at == std::string::npos) return false;
+  char* end = nullptr;
+  *v = std::strtod(line.c_str() + at + tag.size(), &end);
+  return end != line.c_str() + at + tag.size();
+}
+
+// The cached result for (device, cus, key), if any.
+bool tune_lookup(const char* path, const std::string& dev, int cus, const std::string& key, rtg_tune_result* r) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) return false;
+  const std::string want = "{\"device\":\"" + dev + "\",\"cus\":" + std::to_string(cus) + ",\"key\":\"" + key + "\",";
+  char buf[1024];
+  bool found = false;
+  while (std::fgets(buf, sizeof buf, f)) {
+    const std::string line(buf);
+    if (line.compare(0, want.size(), want) != 0) continue;
+    double c, w, l, d, b;
+    if (tune_field(line, "chunk_paths", &c) && tune_field(line, "wide_paths", &w) && tune_field(line, "lanes", &l) &&
+        tune_field(line, "default_ms", &d) && tune_field(line, "best_ms", &b)) {
+      *r = rtg_tune_result{};
+      r->best.chunk_paths = (uint32_t)c;
+      r->best.wide_paths = (uint32_t)w;
+      r->best.lanes = (uint32_t)l;
+      r->default_ms = d;
+      r->best_ms = b;
+      found = true;  // the last entry wins
+    }
+  }
+  std::fclose(f);
+  return found;
+}
+
+}  // namespace
+
+extern "C" {
+
+rtg_status rtg_autotune(rtg_context* ctx, const rtg_job* jobs, const uint32_t* call_jobs, uint32_t n_calls,
+                        const char* cache_path, const char* key, rtg_tune_result* out) {
+  if (!ctx || !jobs || !call_jobs || !n_calls || !out) return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_autotune: NULL argument");
+  const std::string k = key ? key : "";
+  if (k.find_first_of("\"\\\n") != std::string::npos)
+    return fail(ctx, RTG_ERR_INVALID_ARG, "rtg_autotune: the key must not hold quotes, backslashes or newlines");
+  hipDeviceProp_t prop;
+  HIP_TRY(ctx, hipGetDeviceProperties(&prop, ctx->device));
+  const std::string dev = prop.gcnArchName;
+  if (cache_path && tune_lookup(cache_path, dev, prop.multiProcessorCount, k, out)) {
+    if (rtg_status st = rtg_set_tuning(ctx, &out->best)) return st;
+    return RTG_OK;
+  }
+  std::vector<rtg_accum*> accs;
+  uint64_t total = 0;
+  for (uint32_t c = 0; c < n_calls; ++c) total += call_jobs[c];
+  for (uint64_t i = 0; i < total; ++i)
+    if (std::find(accs.begin(), accs.end(), jobs[i].acc) == accs.end()) accs.push_back(jobs[i].acc);
+  // Wall time of the whole workload under `t`.
+  auto time_with = [&](const rtg_tuning& t, double* ms) -> rtg_status {
+    if (rtg_status st = rtg_set_tuning(ctx, &t)) return st;
+    for (rtg_accum* a : accs)
+      if (rtg_status st = rtg_accum_clear(ctx, a)) return st;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    uint64_t at = 0;
+    for (uint32_t c = 0; c < n_calls; ++c) {
+      if (rtg_status st = rtg_render_batch(ctx, jobs + at, call_jobs[c], nullptr)) return st;
+      at += call_jobs[c];
+    }
+    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return RTG_OK;
+  };
+  const rtg_tuning saved = ctx->tuning;
+  rtg_tuning best{};
+  double best_ms = 0, warm = 0;
+  uint32_t trials = 0;
+  rtg_status st = time_with(best, &warm);  // grows the buffers; not compared
+  if (st == RTG_OK) st = time_with(best, &best_ms);
+  const double default_ms = best_ms;
+  // Coordinate descent: each knob in turn over its candidates, the others
+  // held at the best found so far.
+  const uint32_t chunks[] = {1u << 23, 1u << 24, 1u << 25, 1u << 26};
+  const uint32_t wides[] = {1u << 20, 1u << 22, 1u << 24, 0xFFFFFFFFu};
+  const uint32_t lanes[] = {1, 2};
+  for (int knob = 0; knob < 3 && st == RTG_OK; ++knob) {
+    const uint32_t* cand = knob == 0 ? chunks : knob == 1 ? wides : lanes;
+    const int n = knob == 2 ? 2 : 4;
+    for (int c = 0; c < n && st == RTG_OK; ++c) {
+      rtg_tuning t = best;
+      (knob == 0 ? t.chunk_paths : knob == 1 ? t.wide_paths : t.lanes) = cand[c];
+      const uint32_t effective = knob == 0 ? (best.chunk_paths ? best.chunk_paths : kDefaultChunkPaths)
+                                 : knob == 1 ? (best.wide_paths ? best.wide_paths : (uint32_t)kDefaultWidePaths)
+                                             : (best.lanes ? best.lanes : 2u);
+      if (cand[c] == effective) continue;  // the incumbent, already timed
+      double ms = 0;
+      st = time_with(t, &ms);
+      ++trials;
+      if (st == RTG_OK && ms < best_ms) {
+        best = t;
+        best_ms = ms;
+      }
+    }
+  }
+  for (rtg_accum* a : accs) rtg_accum_clear(ctx, a);
+  if (st != RTG_OK) {
+    ctx->tuning = saved;
+    return st;
+  }
+  // Re-time the default and the winner side by side for the report.
+  double d2 = 0, b2 = 0;
+  if ((st = time_with(rtg_tuning{}, &d2)) != RTG_OK || (st = time_with(best, &b2)) != RTG_OK) {
+    ctx->tuning = saved;
+    return st;
+  }
+  for (rtg_accum* a : accs) rtg_accum_clear(ctx, a);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  *out = rtg_tune_result{};
+  rtg_set_tuning(ctx, &best);
+  rtg_get_tuning(ctx, &out->best);
+  out->default_ms = std::min(default_ms, d2);
+  out->best_ms = std::min(best_ms, b2);
+  out->trials = trials;
+  if (cache_path) {
+    if (FILE* f = std::fopen(cache_path, "a")) {
+      std::fprintf(f, "%s\n", tune_line(dev, prop.multiProcessorCount, k, *out).c_str());
+      std::fclose(f);
+    }
+  }
   return RTG_OK;
 }
 

@@ -89,6 +89,19 @@ class Job(C.Structure):
                 ("s0", C.c_uint32), ("s1", C.c_uint32), ("p0", C.c_uint32), ("p1", C.c_uint32)]
 
 
+class Tuning(C.Structure):
+    """rtg_tuning: scheduling knobs (0 = built-in default)."""
+    _fields_ = [("chunk_paths", C.c_uint32), ("wide_paths", C.c_uint32), ("lanes", C.c_uint32), ("pad", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("best", Tuning), ("default_ms", C.c_double), ("best_ms", C.c_double), ("trials", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
 class Hit(C.Structure):
     _fields_ = [("t", C.c_double), ("prim", C.c_uint32), ("flags", C.c_uint32)]
 
@@ -158,6 +171,10 @@ def load_rtg():
         "rtg_trace": (C.c_int32, [_VP, _VP, _VP, C.c_uint64, P(TraceOpts), _VP, P(Stats)]),
         "rtg_trace_host": (C.c_int32, [_VP, _VP, _VP, C.c_uint64, P(TraceOpts), _VP, P(Stats)]),
         "rtg_synchronize": (C.c_int32, [_VP]),
+        "rtg_get_tuning": (C.c_int32, [_VP, P(Tuning)]),
+        "rtg_set_tuning": (C.c_int32, [_VP, P(Tuning)]),
+        "rtg_autotune": (C.c_int32, [_VP, P(Job), P(C.c_uint32), C.c_uint32, C.c_char_p, C.c_char_p,
+                                     P(TuneResult)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -175,7 +192,8 @@ def rtg_symbols():
             "rtg_render_pass_static", "rtg_resolve",
             "rtg_buffer_create",
             "rtg_buffer_release", "rtg_buffer_write", "rtg_buffer_read", "rtg_trace", "rtg_trace_host",
-            "rtg_synchronize", "rtg_set_profiling", "rtg_set_profiling_kinds", "rtg_get_profile"]
+            "rtg_synchronize", "rtg_set_profiling", "rtg_set_profiling_kinds", "rtg_get_profile",
+            "rtg_get_tuning", "rtg_set_tuning", "rtg_autotune"]
 
 
 def _ptr(a):
@@ -285,6 +303,31 @@ class Context:
         st = (Stats * max(1, len(jobs)))()
         self._check(self.lib.rtg_render_batch(self.h, arr, len(jobs), st))
         return list(st)[:len(jobs)]
+
+    def get_tuning(self):
+        t = Tuning()
+        self._check(self.lib.rtg_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, chunk_paths=0, wide_paths=0, lanes=0):
+        """Scheduling knobs (rtg_set_tuning); 0 = built-in default."""
+        self._check(self.lib.rtg_set_tuning(self.h, C.byref(Tuning(chunk_paths, wide_paths, lanes, 0))))
+
+    def autotune(self, calls, cache_path=None, key="default"):
+        """rtg_autotune over a workload of batch calls: calls = list of job
+        lists as render_batch takes them. Returns a dict with the chosen
+        tuning, the default and tuned wall times and the trials timed."""
+        flat = [j for call in calls for j in call]
+        arr = (Job * max(1, len(flat)))()
+        for i, j in enumerate(flat):
+            scene, acc, cfg, s0, s1 = j[:5]
+            p0, p1 = (j[5], j[6]) if len(j) > 5 else (0, 0)
+            arr[i] = Job(scene.h.value, acc.h.value, cfg, s0, s1, p0, p1)
+        sizes = (C.c_uint32 * len(calls))(*[len(c) for c in calls])
+        r = TuneResult()
+        self._check(self.lib.rtg_autotune(self.h, arr, sizes, len(calls),
+                                          cache_path.encode() if cache_path else None, key.encode(), C.byref(r)))
+        return {"tuning": r.best.as_dict(), "default_ms": r.default_ms, "best_ms": r.best_ms, "trials": r.trials}
 
     def trace(self, scene, rays: np.ndarray, mode=0, use_bvh=1, precision=PRECISION_F64):
         rays = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 8)

@@ -170,6 +170,38 @@ def test_graph_replays_equal_direct_launches(ctx, specs, monkeypatch):
                 assert getattr(a, k) == getattr(b, k), (rnd, i, k)
 
 
+def test_autotune_keeps_images_and_caches_per_device(ctx, specs, tmp_path):
+    """The autotuner times candidate knobs on a workload of batch calls; the
+    images do not depend on the knobs it picks, and a second call with the
+    same cache file and key reuses the result without timing."""
+    devs = [ctx.scene(o.desc) for o, _, _ in specs]
+    accs = [ctx.accum(cfg.width, cfg.height) for _, cfg, _ in specs]
+    calls = [[(d, a, cfg, 0, 1) for d, a, (o, cfg, spp) in zip(devs, accs, specs)],
+             [(d, a, cfg, 1, spp) for d, a, (o, cfg, spp) in zip(devs, accs, specs) if spp > 1]]
+    ctx.set_tuning()
+    for call in calls:
+        ctx.render_batch(call)
+    ref = [a.read() for a in accs]
+    cache = str(tmp_path / "tune.jsonl")
+    r1 = ctx.autotune(calls, cache_path=cache, key="small-mix")
+    print("autotune", r1, flush=True)
+    assert r1["trials"] > 0 and r1["best_ms"] > 0 and r1["default_ms"] > 0
+    assert ctx.get_tuning() == r1["tuning"]
+    assert all(int(a.read()[1].max()) == 0 for a in accs)  # left cleared
+    for call in calls:
+        ctx.render_batch(call)
+    for (s, c), (s0, c0) in zip((a.read() for a in accs), ref):
+        assert np.array_equal(c, c0) and np.array_equal(s.view(np.uint64), s0.view(np.uint64))
+    ctx.set_tuning()
+    r2 = ctx.autotune(calls, cache_path=cache, key="small-mix")
+    assert r2["trials"] == 0 and r2["tuning"] == r1["tuning"] and ctx.get_tuning() == r1["tuning"]
+    with pytest.raises(rtg.RtgError, match="key"):
+        ctx.autotune(calls, cache_path=cache, key='bad"key')
+    ctx.set_tuning()
+    for x in accs + devs:
+        x.release()
+
+
 def test_bad_jobs_are_rejected(ctx, specs):
     o, cfg, _ = specs[0]
     dev = ctx.scene(o.desc)
