@@ -259,19 +259,21 @@ def batch_leg(rtg, seed=4, count=64):
             "identical": identical and same_q and rays_p == rays_b == rays_q}
 
 
-def serve_leg(rtg, qps=6.0, seconds=10.0, seed=5, burst=300, tick_paths=0):
-    """The served workload: Poisson arrivals at `qps` for `seconds` over 16
-    resident scenes of the batched workload. 80 % previews (128^2, 1-4 spp,
-    depth 4, one pass), 20 % finals (512^2, 768^2 or 1024^2, 256 spp, the
-    scene's depth, progressive passes capped at 32). The same trace runs
-    through the served loop and the plain loop; then `burst` previews arriving
-    at once measure saturation throughput."""
+def serve_leg(rtg, seconds=3.0, loads=(0.5, 0.8, 1.0), seed=5, burst=300, sat_requests=60, tick_paths=0):
+    """The served workload over 16 resident scenes of the batched workload:
+    80 % previews (128^2, 1-4 spp, depth 4, one pass), 20 % finals (512^2,
+    768^2 or 1024^2, 256 spp, the scene's depth, progressive passes capped at
+    32). First the saturation rate: `sat_requests` mixed requests arriving at
+    once through the plain loop (one request at a time). Then Poisson traces
+    of `seconds` at each fraction of that rate in `loads`, each through the
+    served loop and the plain loop; then `burst` previews arriving at once.
+    Every request's image hash must agree between the loops."""
     import numpy as np
     sys.path.insert(0, os.path.join(PKG, "scenes"))
     import make_batch
     specs = make_batch.build(seed, 16)
     hs = [rtg.HostScene(json.dumps(j)) for j in specs]
-    depth = [int(j["render"]["max_depth"]) for j in specs]
+    depth = np.array([int(j["render"]["max_depth"]) for j in specs])
     srv = rtg.Server(0, 0, tick_paths)
     for h in hs:
         srv.add_scene(h)
@@ -286,37 +288,65 @@ def serve_leg(rtg, qps=6.0, seconds=10.0, seed=5, burst=300, tick_paths=0):
         size = rng.choice([512, 768, 1024], n)
         r["width"] = r["height"] = np.where(final, size, 128)
         r["spp"] = np.where(final, 256, rng.integers(1, 5, n))
-        r["max_depth"] = np.where(final, np.array(depth)[r["scene"]], 4)
+        r["max_depth"] = np.where(final, depth[r["scene"]], 4)
         r["max_pass"] = np.where(final, 32, 4)
         return r
+
+    def summary(reqs, r, st):
+        lat = r["done_ms"] - r["arrival_ms"]
+        prev, fin = reqs["final_kind"] == 0, reqs["final_kind"] == 1
+        return {"preview_p50_ms": float(np.percentile(lat[prev], 50)) if prev.any() else None,
+                "preview_p99_ms": float(np.percentile(lat[prev], 99)) if prev.any() else None,
+                "final_p50_ms": float(np.percentile(lat[fin], 50)) if fin.any() else None,
+                "final_first_frame_p50_ms": float(np.percentile((r["first_frame_ms"] - r["arrival_ms"])[fin], 50))
+                if fin.any() else None,
+                "wall_s": st["wall_ms"] / 1e3, "requests_per_s": len(reqs) / (st["wall_ms"] / 1e3),
+                "g_rays_per_s": st["rays"] / st["wall_ms"] / 1e6, "gpu_busy": st["gpu_ms"] / st["wall_ms"],
+                "ticks": st["ticks"], "max_active": st["max_active"],
+                "peak_footprint_gib": st["peak_footprint_bytes"] / 2 ** 30}
 
     warm = requests(4, np.zeros(4), np.array([1, 0, 0, 0]))
     warm["width"][0] = warm["height"][0] = 1024
     srv.run(warm, served=True)  # grows the path buffers before timing
     srv.run(warm, served=False)
-    n = int(rng.poisson(qps * seconds))
-    reqs = requests(n, np.sort(rng.uniform(0.0, seconds * 1e3, n)), rng.random(n) < 0.2)
-    out = {"trace": f"{n} requests in {seconds:.0f} s (Poisson, {qps:g}/s): {int(reqs['final_kind'].sum())} finals, "
-                    f"{int((reqs['final_kind'] == 0).sum())} previews over 16 resident scenes"}
-    res = {}
+    out = {"scenes": "16 resident scenes of the batched workload (seed 5)",
+           "mix": "80 % previews 128^2 1-4 spp depth 4; 20 % finals 512^2-1024^2 256 spp progressive"}
+    # Saturation: the mixed requests all at once; the plain loop's rate is
+    # the capacity the load levels below are fractions of.
+    sat = requests(sat_requests, np.zeros(sat_requests), rng.random(sat_requests) < 0.2)
+    sat_out = {}
     for name, served in (("served", True), ("plain", False)):
-        r, st = srv.run(reqs, served=served)
-        lat = r["done_ms"] - r["arrival_ms"]
-        prev, fin = reqs["final_kind"] == 0, reqs["final_kind"] == 1
-        res[name] = r
-        out[name] = {"preview_p50_ms": float(np.percentile(lat[prev], 50)),
-                     "preview_p99_ms": float(np.percentile(lat[prev], 99)),
-                     "final_p50_ms": float(np.percentile(lat[fin], 50)) if fin.any() else None,
-                     "final_first_frame_p50_ms": float(np.percentile((r["first_frame_ms"] - r["arrival_ms"])[fin], 50))
-                     if fin.any() else None,
-                     "wall_s": st["wall_ms"] / 1e3, "g_rays_per_s": st["rays"] / st["wall_ms"] / 1e6,
-                     "ticks": st["ticks"], "max_active": st["max_active"]}
-    out["identical"] = bool(np.array_equal(res["served"]["image_hash"], res["plain"]["image_hash"]))
+        r, st = srv.run(sat, served=served)
+        sat_out[name] = summary(sat, r, st)
+        sat_out[name]["hash"] = r["image_hash"]
+    rate = sat_out["plain"]["requests_per_s"]
+    out["saturation"] = {"requests": sat_requests, "plain_requests_per_s": rate,
+                         "served_requests_per_s": sat_out["served"]["requests_per_s"],
+                         "served_vs_plain": sat_out["served"]["requests_per_s"] / rate,
+                         "identical": bool(np.array_equal(sat_out["served"].pop("hash"), sat_out["plain"].pop("hash")))}
+    levels = []
+    identical = out["saturation"]["identical"]
+    for load in loads:
+        n = max(5, int(rng.poisson(load * rate * seconds)))
+        reqs = requests(n, np.sort(rng.uniform(0.0, seconds * 1e3, n)), rng.random(n) < 0.2)
+        lv = {"load": load, "qps": load * rate, "requests": n, "finals": int(reqs["final_kind"].sum())}
+        res = {}
+        for name, served in (("served", True), ("plain", False)):
+            r, st = srv.run(reqs, served=served)
+            res[name] = r
+            lv[name] = summary(reqs, r, st)
+        lv["identical"] = bool(np.array_equal(res["served"]["image_hash"], res["plain"]["image_hash"]))
+        identical = identical and lv["identical"]
+        levels.append(lv)
+    out["levels"] = levels
     pb = requests(burst, np.zeros(burst), np.zeros(burst, bool))
+    bursts = {}
     for name, served in (("served", True), ("plain", False)):
         r, st = srv.run(pb, served=served)
-        out[name]["burst_previews_per_s"] = burst / (st["wall_ms"] / 1e3)
-    out["preview_throughput_ratio"] = out["served"]["burst_previews_per_s"] / out["plain"]["burst_previews_per_s"]
+        bursts[name] = burst / (st["wall_ms"] / 1e3)
+    out["preview_burst"] = {"previews": burst, "served_per_s": bursts["served"], "plain_per_s": bursts["plain"],
+                            "ratio": bursts["served"] / bursts["plain"]}
+    out["identical"] = identical
     srv.close()
     return out
 
@@ -334,8 +364,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 (reference-precision) renders")
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
-    ap.add_argument("--qps", type=float, default=6.0, help="served workload: request arrival rate")
-    ap.add_argument("--serve-seconds", type=float, default=10.0, help="served workload: trace length")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="served workload: trace length per load level")
+    ap.add_argument("--serve-loads", default="0.5,0.8,1.0",
+                    help="served workload: arrival rates as fractions of the plain loop's saturation rate")
     ap.add_argument("--tick-paths", type=int, default=0,
                     help="served workload: final-render paths per tick (0: tuned per tick)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -435,7 +466,8 @@ def main():
     acc.release()
     ctx.close()
     batch = None if args.no_batch else batch_leg(rtg)
-    served = None if args.no_serve else serve_leg(rtg, qps=args.qps, seconds=args.serve_seconds,
+    served = None if args.no_serve else serve_leg(rtg, seconds=args.serve_seconds,
+                                                     loads=tuple(float(x) for x in args.serve_loads.split(",")),
                                                      tick_paths=args.tick_paths)
     cpu = None
     if not args.no_cpu:

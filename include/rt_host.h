@@ -110,7 +110,9 @@ int rth_render(rth_scene* s, int32_t device, int32_t precision, int32_t progress
  * all samples of their pass in the next tick; finals advance their
  * progressive passes (1, 1, 2, 4, ... capped at max_pass, renderer.cpp:66-69)
  * in slices sized per tick (see rts_create); requests are admitted first come
- * first served while their accumulation buffers fit budget_bytes. It has no
+ * first served while the HBM footprint — resident scenes, the path state and
+ * queues of the server's context, the admitted requests' accumulation
+ * buffers — fits budget_bytes. It has no
  * counterpart in the reference, whose editor backend starts one `rt` process
  * per job (web/backend/jobs.go:130-259). Errors: 1 + rth_last_error(). */
 typedef struct rts_server rts_server;
@@ -137,11 +139,17 @@ typedef struct rts_stats {
   uint64_t rays; /* closest-hit + shadow rays */
   uint32_t ticks, max_active;
   uint64_t peak_accum_bytes;
+  double gpu_ms;                 /* wall time inside render calls */
+  uint64_t fixed_bytes;          /* resident scenes + path state and queues, at the end */
+  uint64_t peak_footprint_bytes; /* the largest footprint admission saw */
 } rts_stats;
 
 /* budget_bytes 0: 64 GiB; tick_paths: final-render paths per tick, 0 = tuned
- * (each tick aimed at 12 ms from the scenes' measured cost per path). */
+ * (each tick aimed at 7 ms with previews in it, 20 ms without, from the
+ * scenes' measured cost per path). */
 int rts_create(int32_t device, uint64_t budget_bytes, uint32_t tick_paths, rts_server** out);
+/* Replaces the footprint budget (0: 64 GiB) for the next rts_run. */
+int rts_set_budget(rts_server* s, uint64_t budget_bytes);
 /* Uploads a compiled scene (kept resident); its render settings supply the
  * tone map, exposure, roulette depth and light sampling of every request. */
 int rts_add_scene(rts_server* s, const rth_scene* scene, int32_t* index);

@@ -195,6 +195,11 @@ rtg_status rtg_scene_with_camera(rtg_context* ctx, const rtg_scene* base, const 
                                  rtg_scene** out);
 /* Device bytes held by the scene (admission by HBM footprint). */
 uint64_t rtg_scene_bytes(const rtg_scene* scene);
+/* Device bytes the context holds now: path state and queues of every lane
+ * (grown on demand to the largest chunk or batch group seen), statistics
+ * slots, batch tables, resolve staging. With rtg_scene_bytes and the
+ * accumulation buffers, the HBM footprint of a render service. */
+uint64_t rtg_context_bytes(const rtg_context* ctx);
 
 /* Accumulation buffer: per-pixel FP64 sum + u32 count (the reference
  * Framebuffer, rendering/framebuffer.hpp:16-50), resident on the device. */

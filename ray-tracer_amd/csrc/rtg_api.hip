@@ -706,6 +706,18 @@ rtg_status rtg_scene_release(rtg_scene* s) {
 
 uint64_t rtg_scene_bytes(const rtg_scene* s) { return s ? s->bytes : 0; }
 
+uint64_t rtg_context_bytes(const rtg_context* ctx) {
+  if (!ctx) return 0;
+  uint64_t b = 0;
+  for (uint64_t x : ctx->path_bytes) b += x;
+  b += sizeof(unsigned long long) * kStatSlots * kStatStride;                   // stats
+  b += sizeof(unsigned long long) * kStatSlots * kStatStride * ctx->job_stats_cap;  // single-render job stats
+  for (int l = 0; l < 2; ++l)
+    if (ctx->batch_jobs[l]) b += sizeof(BatchJob) * kMaxBatchJobs + sizeof(uint32_t) * (1 + 5 * (kMaxBatchJobs + 1));
+  b += ctx->rgb_cap;
+  return b;
+}
+
 // ---------------------------------------------------------------------------
 rtg_status rtg_accum_create(rtg_context* ctx, int32_t w, int32_t h, rtg_accum** out) {
   if (!ctx || !out || w < 1 || h < 1 || w > 16384 || h > 16384)

@@ -259,12 +259,23 @@ int rts_run(rts_server* s, const rts_request* reqs, uint32_t n, int32_t served, 
       results[i] = rts_result{r.arrival_ms, r.start_ms, r.first_frame_ms, r.done_ms, r.rays,
                               r.shadow_rays, r.frames,   r.spp_done,       r.image_hash};
     }
-    if (stats) *stats = rts_stats{st.wall_ms, st.rays, st.ticks, st.max_active, st.peak_accum_bytes};
+    if (stats)
+      *stats = rts_stats{st.wall_ms,  st.rays,           st.ticks,      st.max_active, st.peak_accum_bytes,
+                         st.gpu_ms,   st.fixed_bytes,    st.peak_footprint_bytes};
     return 0;
   } catch (const std::exception& e) {
     g_last_error = e.what();
     return 1;
   }
+}
+
+int rts_set_budget(rts_server* s, uint64_t budget_bytes) {
+  if (!s) {
+    g_last_error = "rts_set_budget: NULL server";
+    return 1;
+  }
+  s->server.set_budget(budget_bytes);
+  return 0;
 }
 
 void rts_free(rts_server* s) { delete s; }

@@ -145,7 +145,9 @@ class Service {
     std::vector<rtg_job> batch;
     std::vector<Job*> owner;
     std::vector<uint32_t> pixels;
-    tuner_.begin();
+    bool previews = false;
+    for (const Job& j : jobs_) previews = previews || j.preview;
+    tuner_.begin(previews ? kTickTargetMs : kFinalsTickTargetMs);
     for (int kind = 0; kind < 2; ++kind)  // previews first, whole
       for (Job& j : jobs_) {
         if (j.preview != (kind == 0)) continue;

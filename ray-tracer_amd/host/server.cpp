@@ -3,8 +3,11 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -41,6 +44,60 @@ uint32_t pass_samples(uint32_t done, uint32_t spp, uint32_t max_pass) {
 
 }  // namespace
 
+// Hashes final images on a worker thread, so the GPU goes on with the next
+// tick (or request) while the host reduces a read-back buffer.
+class Server::Hasher {
+ public:
+  Hasher() : worker_([this] { loop(); }) {}
+  ~Hasher() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    worker_.join();
+  }
+  void submit(uint64_t* dst, std::vector<double> sum, std::vector<uint32_t> count) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      work_.push_back(Item{dst, std::move(sum), std::move(count)});
+    }
+    cv_.notify_all();
+  }
+  // Returns once every submitted image is hashed.
+  void drain() {
+    std::unique_lock<std::mutex> lk(mu_);
+    idle_.wait(lk, [&] { return work_.empty() && !busy_; });
+  }
+
+ private:
+  struct Item {
+    uint64_t* dst;
+    std::vector<double> sum;
+    std::vector<uint32_t> count;
+  };
+  void loop() {
+    std::unique_lock<std::mutex> lk(mu_);
+    while (true) {
+      cv_.wait(lk, [&] { return stop_ || !work_.empty(); });
+      if (work_.empty()) return;
+      Item it = std::move(work_.front());
+      work_.pop_front();
+      busy_ = true;
+      lk.unlock();
+      *it.dst = fnv1a_words(fnv1a_words(1469598103934665603ull, it.sum), it.count);
+      lk.lock();
+      busy_ = false;
+      idle_.notify_all();
+    }
+  }
+  std::mutex mu_;
+  std::condition_variable cv_, idle_;
+  std::deque<Item> work_;
+  bool stop_ = false, busy_ = false;
+  std::thread worker_;
+};
+
 Server::Server(int device, uint64_t budget_bytes, uint32_t tick_paths)
     : budget_(budget_bytes ? budget_bytes : (64ull << 30)), tuner_(tick_paths, kTickTargetMs) {
   rtg_check(rtg_open(device, &ctx_), nullptr, "rtg_open");
@@ -61,6 +118,7 @@ int Server::add_scene(const CompiledScene& c) {
   slot.camera = c.camera_desc;
   const rtg_scene_desc v = c.view();
   rtg_check(rtg_scene_create(ctx_, &v, &slot.dev), ctx_, "rtg_scene_create");
+  scene_bytes_ += rtg_scene_bytes(slot.dev);
   scenes_.push_back(std::move(slot));
   return (int)scenes_.size() - 1;
 }
@@ -122,12 +180,14 @@ void Server::frame(rtg_accum* acc, const ServeRequest& r) {
   rtg_check(rtg_resolve(ctx_, acc, (int32_t)R.tone_map, R.exposure, rgb_.data()), ctx_, "rtg_resolve");
 }
 
-uint64_t Server::finish(rtg_accum* acc, const ServeRequest& r) {
+uint64_t Server::fixed_bytes() const { return scene_bytes_ + rtg_context_bytes(ctx_); }
+
+void Server::finish(rtg_accum* acc, const ServeRequest& r, uint64_t* hash) {
   const size_t n = (size_t)r.width * r.height;
   std::vector<double> sum(n * 3);
   std::vector<uint32_t> count(n);
   rtg_check(rtg_accum_read(ctx_, acc, sum.data(), count.data()), ctx_, "rtg_accum_read");
-  return fnv1a_words(fnv1a_words(1469598103934665603ull, sum), count);
+  hasher_->submit(hash, std::move(sum), std::move(count));
 }
 
 ServeStats Server::run(const std::vector<ServeRequest>& reqs, bool served, std::vector<ServeResult>& out) {
@@ -143,7 +203,18 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, bool served, std::
   out.assign(n, ServeResult{});
   for (const ServeRequest& r : reqs) scene_for(r);  // camera set-ups before the clock starts
   ServeStats st;
-  uint64_t used = 0;
+  uint64_t used = 0;  // accumulation buffers of admitted requests
+  Hasher hasher;
+  hasher_ = &hasher;
+  struct Detach {
+    Server* s;
+    ~Detach() { s->hasher_ = nullptr; }
+  } detach{this};
+  auto gpu_call = [&](auto&& f) {
+    const auto c0 = Clock::now();
+    f();
+    st.gpu_ms += ms_since(c0);
+  };
   const auto t0 = Clock::now();
 
   auto complete_pass = [&](size_t i, rtg_accum* acc, uint32_t k) {
@@ -169,13 +240,16 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, bool served, std::
         ~Back() { s->recycle(a, r); }
       } back{this, acc, r};
       st.peak_accum_bytes = std::max(st.peak_accum_bytes, accum_bytes(r));
+      st.peak_footprint_bytes = std::max(st.peak_footprint_bytes, fixed_bytes() + accum_bytes(r));
       st.max_active = 1;
       rtg_scene* sc = scene_for(r);
       const rtg_render_settings cfg = settings_of(r);
       for (uint32_t done = 0; done < (uint32_t)r.spp;) {
         const uint32_t k = pass_samples(done, (uint32_t)r.spp, (uint32_t)r.max_pass);
         rtg_stats s{};
-        rtg_check(rtg_render_pass(ctx_, sc, &cfg, done, done + k, acc, nullptr, &s), ctx_, "rtg_render_pass");
+        gpu_call([&] {
+          rtg_check(rtg_render_pass(ctx_, sc, &cfg, done, done + k, acc, nullptr, &s), ctx_, "rtg_render_pass");
+        });
         o.rays += s.rays;
         o.shadow_rays += s.shadow_rays;
         ++st.ticks;
@@ -183,10 +257,12 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, bool served, std::
         complete_pass(i, acc, k);
       }
       o.done_ms = ms_since(t0);
-      o.image_hash = finish(acc, r);
+      finish(acc, r, &o.image_hash);
       st.rays += o.rays + o.shadow_rays;
     }
     st.wall_ms = ms_since(t0);
+    hasher.drain();
+    st.fixed_bytes = fixed_bytes();
     return st;
   }
 
@@ -207,10 +283,14 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, bool served, std::
     const double now = ms_since(t0);
     while (next < n && reqs[next].arrival_ms <= now) {
       const ServeRequest& r = reqs[next];
-      if (!live.empty() && used + accum_bytes(r) > budget_) break;  // admission by memory footprint
+      // Admission by HBM footprint: resident scenes, path state and queues,
+      // the admitted requests' accumulation buffers.
+      const uint64_t footprint = fixed_bytes() + used + accum_bytes(r);
+      if (!live.empty() && footprint > budget_) break;
       rtg_accum* acc = acquire(r);
       used += accum_bytes(r);
       st.peak_accum_bytes = std::max(st.peak_accum_bytes, used);
+      st.peak_footprint_bytes = std::max(st.peak_footprint_bytes, footprint);
       out[next].arrival_ms = r.arrival_ms;
       out[next].start_ms = now;
       live.push_back(Live{next, acc, pass_samples(0, (uint32_t)r.spp, (uint32_t)r.max_pass), 0});
@@ -226,7 +306,9 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, bool served, std::
     jobs.clear();
     job_pixels.clear();
     job_live.clear();
-    tuner_.begin();
+    bool previews = false;
+    for (const Live& L : live) previews = previews || reqs[L.i].final_kind == 0;
+    tuner_.begin(previews ? kTickTargetMs : kFinalsTickTargetMs);
     for (int pass_kind = 0; pass_kind < 2; ++pass_kind)
       for (size_t li = 0; li < live.size(); ++li) {
         const Live& L = live[li];
@@ -259,6 +341,7 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, bool served, std::
     rtg_check(rtg_render_batch(ctx_, jobs.data(), (uint32_t)jobs.size(), js.data()), ctx_, "rtg_render_batch");
     const double tick_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tick0).count();
+    st.gpu_ms += tick_ms;
     ++st.ticks;
     for (size_t q = 0; q < jobs.size(); ++q) {
       const ServeRequest& rq = reqs[live[job_live[q]].i];
@@ -281,7 +364,7 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, bool served, std::
         continue;
       }
       o.done_ms = ms_since(t0);
-      o.image_hash = finish(L.acc, r);
+      finish(L.acc, r, &o.image_hash);
       recycle(L.acc, r);
       L.acc = nullptr;
       used -= accum_bytes(r);
@@ -295,6 +378,8 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, bool served, std::
     throw;
   }
   st.wall_ms = ms_since(t0);
+  hasher.drain();
+  st.fixed_bytes = fixed_bytes();
   return st;
 }
 

@@ -13,8 +13,13 @@
 //     tick tuner (tick_tuner.hpp: each tick aimed at kTickTargetMs from the
 //     scenes' measured cost per path) or by a fixed `tick_paths` paths per
 //     tick; a frame is resolved when a pass ends;
-//   * requests are admitted first-come first-served while the accumulation
-//     buffers of admitted requests fit the memory budget.
+//   * requests are admitted first-come first-served while the HBM footprint
+//     fits the memory budget: the resident scenes (BVH, primitives,
+//     shading data), the context's path state and queues, and the
+//     accumulation buffers of the admitted requests;
+//   * a tick with previews is aimed at kTickTargetMs, a tick of finals only
+//     at kFinalsTickTargetMs (tick_tuner.hpp); final images are read back
+//     when done and hashed on a worker thread while the GPU goes on.
 // A request's image is bitwise the image it gets when rendered alone: its
 // samples are keyed on (seed, pixel, sample, bounce) and every pixel adds its
 // samples in sample order, however passes are sliced and batched.
@@ -52,11 +57,15 @@ struct ServeStats {
   uint64_t rays = 0;  // closest-hit + shadow rays
   uint32_t ticks = 0, max_active = 0;
   uint64_t peak_accum_bytes = 0;
+  double gpu_ms = 0;                  // wall time inside render calls
+  uint64_t fixed_bytes = 0;           // resident scenes + the context's path state at the end
+  uint64_t peak_footprint_bytes = 0;  // the largest footprint admission saw
 };
 
 class Server {
  public:
-  // budget_bytes: accumulation buffers of admitted requests (0: 64 GiB);
+  // budget_bytes: HBM footprint (resident scenes, path state, accumulation
+  // buffers of admitted requests; 0: 64 GiB);
   // tick_paths: paths of final renders per tick (0: tuned, each tick aimed
   // at kTickTargetMs).
   Server(int device, uint64_t budget_bytes, uint32_t tick_paths);
@@ -71,6 +80,9 @@ class Server {
   // through the single-render path). out[i] belongs to reqs[i].
   ServeStats run(const std::vector<ServeRequest>& reqs, bool served, std::vector<ServeResult>& out);
   static uint64_t accum_bytes(const ServeRequest& r) { return (uint64_t)r.width * r.height * 28; }
+  void set_budget(uint64_t budget_bytes) { budget_ = budget_bytes ? budget_bytes : (64ull << 30); }
+  // Resident scenes and the context's path state and queues now.
+  uint64_t fixed_bytes() const;
 
  private:
   struct SceneSlot {
@@ -85,11 +97,16 @@ class Server {
   // the device) instead of being allocated per request.
   rtg_accum* acquire(const ServeRequest& r);
   void recycle(rtg_accum* acc, const ServeRequest& r);
-  uint64_t finish(rtg_accum* acc, const ServeRequest& r);  // hash of the final buffer
+  // Reads the final buffer back; its hash is computed on the worker thread
+  // and written to *hash before run() returns.
+  void finish(rtg_accum* acc, const ServeRequest& r, uint64_t* hash);
   void frame(rtg_accum* acc, const ServeRequest& r);       // tone map + readback of one frame
 
   rtg_context* ctx_ = nullptr;
   std::vector<SceneSlot> scenes_;
+  uint64_t scene_bytes_ = 0;
+  class Hasher;
+  Hasher* hasher_ = nullptr;  // during run()
   uint64_t budget_;
   TickTuner tuner_;
   // Tuner key of a request: its scene and kind (previews and finals differ

@@ -22,6 +22,11 @@ namespace rtm {
 // waits for it and then takes its own, ~2 x 7 ms, well inside a 50 ms
 // preview latency goal.
 constexpr double kTickTargetMs = 7.0;
+// Target of a tick with no preview in it: a preview arriving meanwhile waits
+// at most this long before its own tick, still inside 50 ms, and the finals
+// lose less to tick boundaries (each one a host round trip with the GPU
+// idle).
+constexpr double kFinalsTickTargetMs = 20.0;
 
 class TickTuner {
  public:
@@ -33,9 +38,10 @@ class TickTuner {
 
   bool tuned() const { return fixed_ == 0; }
 
-  // Start of a tick.
-  void begin() {
-    left_ = fixed_ ? (double)fixed_ : target_ms_;
+  // Start of a tick aimed at `target_ms` (0: the tuner's target).
+  void begin(double target_ms = 0) {
+    tick_target_ = target_ms > 0 ? target_ms : target_ms_;
+    left_ = fixed_ ? (double)fixed_ : tick_target_;
     finals_ = false;
   }
   // A preview's whole pass (rendered whatever the budget).
@@ -74,7 +80,7 @@ class TickTuner {
   void forget(const void* scene) { ms_per_path_.erase(scene); }
 
  private:
-  double finals_left() const { return fixed_ || finals_ ? left_ : std::max(left_, 0.25 * target_ms_); }
+  double finals_left() const { return fixed_ || finals_ ? left_ : std::max(left_, 0.25 * tick_target_); }
   double cost(const void* scene) const {
     auto it = ms_per_path_.find(scene);
     return it == ms_per_path_.end() ? fallback_ms_per_path_ : it->second;
@@ -85,6 +91,7 @@ class TickTuner {
   };
   uint32_t fixed_;
   double target_ms_, fallback_ms_per_path_;
+  double tick_target_ = 0;
   double left_ = 0;
   bool finals_ = false;
   std::vector<Obs> obs_;

@@ -152,6 +152,7 @@ def load_rtg():
         "rtg_scene_release": (C.c_int32, [_VP]),
         "rtg_scene_with_camera": (C.c_int32, [_VP, _VP, P(Camera), P(_VP)]),
         "rtg_scene_bytes": (C.c_uint64, [_VP]),
+        "rtg_context_bytes": (C.c_uint64, [_VP]),
         "rtg_accum_create": (C.c_int32, [_VP, C.c_int32, C.c_int32, P(_VP)]),
         "rtg_accum_release": (C.c_int32, [_VP]),
         "rtg_accum_clear": (C.c_int32, [_VP, _VP]),
@@ -187,7 +188,7 @@ def rtg_symbols():
     """Names declared in include/rtg.h (checked by the CPU tests)."""
     return ["rtg_open", "rtg_close", "rtg_last_error", "rtg_abi_version", "rtg_device_info", "rtg_scene_create",
             "rtg_scene_node_records",
-            "rtg_scene_release", "rtg_scene_with_camera", "rtg_scene_bytes", "rtg_accum_create", "rtg_accum_release", "rtg_accum_clear",
+            "rtg_scene_release", "rtg_scene_with_camera", "rtg_scene_bytes", "rtg_context_bytes", "rtg_accum_create", "rtg_accum_release", "rtg_accum_clear",
             "rtg_accum_read", "rtg_accum_min_samples", "rtg_render_pass", "rtg_render_passes", "rtg_render_batch",
             "rtg_render_pass_static", "rtg_resolve",
             "rtg_buffer_create",
@@ -303,6 +304,10 @@ class Context:
         st = (Stats * max(1, len(jobs)))()
         self._check(self.lib.rtg_render_batch(self.h, arr, len(jobs), st))
         return list(st)[:len(jobs)]
+
+    def device_bytes(self):
+        """rtg_context_bytes: device memory the context holds now."""
+        return self.lib.rtg_context_bytes(self.h)
 
     def get_tuning(self):
         t = Tuning()
@@ -443,7 +448,8 @@ RESULT_DTYPE = np.dtype([("arrival_ms", "<f8"), ("start_ms", "<f8"), ("first_fra
 
 class ServeStats(C.Structure):
     _fields_ = [("wall_ms", C.c_double), ("rays", C.c_uint64), ("ticks", C.c_uint32), ("max_active", C.c_uint32),
-                ("peak_accum_bytes", C.c_uint64)]
+                ("peak_accum_bytes", C.c_uint64), ("gpu_ms", C.c_double), ("fixed_bytes", C.c_uint64),
+                ("peak_footprint_bytes", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -483,6 +489,7 @@ def load_host():
         "rts_create": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint32, P(_VP)]),
         "rts_add_scene": (C.c_int, [_VP, _VP, P(C.c_int32)]),
         "rts_run": (C.c_int, [_VP, _VP, C.c_uint32, C.c_int32, _VP, P(ServeStats)]),
+        "rts_set_budget": (C.c_int, [_VP, C.c_uint64]),
         "rts_free": (None, [_VP]),
     }
     for name, (res, args) in sig.items():
@@ -496,7 +503,7 @@ def host_symbols():
     return ["rt_validate", "rt_message", "rt_threads", "rt_running", "rt_cancel", "rt_start", "rt_wait",
             "rth_scene_load", "rth_scene_free", "rth_last_error", "rth_scene_desc", "rth_scene_settings",
             "rth_scene_info", "rth_scene_bvh_build_ms", "rth_dump_scene", "rth_render", "rts_create",
-            "rts_add_scene", "rts_run", "rts_free"]
+            "rts_add_scene", "rts_run", "rts_set_budget", "rts_free"]
 
 
 class Server:
@@ -527,6 +534,11 @@ class Server:
         if self.lib.rts_run(self.h, _ptr(reqs), len(reqs), 1 if served else 0, _ptr(res), C.byref(st)):
             raise RtgError(self.lib.rth_last_error().decode())
         return res, st.as_dict()
+
+    def set_budget(self, budget_bytes):
+        """Footprint budget for the next run (0: 64 GiB)."""
+        if self.lib.rts_set_budget(self.h, budget_bytes):
+            raise RtgError(self.lib.rth_last_error().decode())
 
     def close(self):
         if self.h:

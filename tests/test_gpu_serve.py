@@ -103,20 +103,34 @@ def test_request_alone_hashes_the_same(resident, ctx):
     assert fnv1a(s, c) == int(mixed["image_hash"][0])
 
 
-def test_admission_by_memory_budget(resident):
-    """A budget of one final's buffer: requests queue, none is lost, and the
-    images are those of an unconstrained run."""
+def test_admission_by_memory_budget(resident, ctx):
+    """Admission counts the HBM footprint: resident scenes, the context's
+    path state and queues, and the admitted requests' accumulation buffers.
+    A budget that leaves room for one final's buffer: requests queue, none is
+    lost, the footprint never exceeds the budget while more than one request
+    is live, and the images are those of an unconstrained run."""
     reqs = trace(seed=5)
     big = rtg.Server(0, 0, 0)
-    small = rtg.Server(0, int(reqs["width"].max() * reqs["height"].max() * 28), 0)
+    small = rtg.Server(0, 0, 0)
     for h in resident:
         big.add_scene(h)
         small.add_scene(h)
     a, sa = big.run(reqs, served=True)
+    _, warm = small.run(reqs, served=True)  # grows the path state to what this trace needs
+    devs = [ctx.scene(h.desc) for h in resident]
+    scene_bytes = sum(d.nbytes() for d in devs)
+    for d in devs:
+        d.release()
+    assert warm["fixed_bytes"] > scene_bytes > 0  # the scenes, and the path state and queues
+    one = int(reqs["width"].max() * reqs["height"].max() * 28)
+    budget = warm["fixed_bytes"] + one
+    small.set_budget(budget)
     b, sb = small.run(reqs, served=True)
     check_complete(reqs, b)
     assert np.array_equal(a["image_hash"], b["image_hash"])
-    assert sb["peak_accum_bytes"] <= reqs["width"].max() * reqs["height"].max() * 28
+    assert sb["peak_accum_bytes"] <= one and sb["peak_footprint_bytes"] <= budget
+    assert sa["peak_accum_bytes"] > one  # unconstrained, more than one final's worth was live at once
+    print("fixed", warm["fixed_bytes"], "peak footprint", sb["peak_footprint_bytes"], flush=True)
     big.close()
     small.close()
 
