@@ -358,6 +358,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="torus_c2", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--integrator", default=None, choices=["path", "direct", "normals"],
+                    help="override the scene's integrator (the headline workload uses path)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--profile-all", action="store_true",
                     help="time every kernel's launches in the timed region (default: the closest-hit kernel's)")
@@ -397,6 +399,8 @@ def main():
     info = hs.info()
     precision = rtg.PRECISION_F64 if args.precision == "f64" else rtg.PRECISION_F32
     cfg = hs.settings(precision)
+    if args.integrator:
+        cfg.integrator = rtg.INTEGRATORS[args.integrator]
     spp = int(info["spp"])
     ctx = rtg.Context(0)
     dev = ctx.scene(hs.desc)
@@ -454,6 +458,7 @@ def main():
     f64 = None
     if precision == rtg.PRECISION_F32 and not args.no_f64:
         cfg64 = hs.settings(rtg.PRECISION_F64)
+        cfg64.integrator = cfg.integrator
         render_once(rtg, ctx, dev, acc, cfg64, spp)
         ctx.synchronize()
         t64 = time.perf_counter()

@@ -132,6 +132,8 @@ struct rtg_context {
   LaunchHooks hooks{};
   char* scratch_dev = nullptr;  // this device's copy of `scratch` (rtg_render_pass_static)
   rtg_tuning tuning{};          // scheduling knobs (zero: built-in default)
+  void* frame_mem = nullptr;    // direct integrator: recursion frames of a chunk
+  uint64_t frame_bytes = 0;
 };
 
 void drop_batch_graphs(rtg_context* ctx) {
@@ -454,6 +456,7 @@ rtg_status rtg_close(rtg_context* ctx) {
   drop_batch_graphs(ctx);
   for (void* p : ctx->path_mem)
     if (p) hipFree(p);
+  if (ctx->frame_mem) hipFree(ctx->frame_mem);
   if (ctx->stats) hipFree(ctx->stats);
   if (ctx->stats_host) hipHostFree(ctx->stats_host);
   if (ctx->job_stats) hipFree(ctx->job_stats);
@@ -715,6 +718,7 @@ uint64_t rtg_context_bytes(const rtg_context* ctx) {
   for (int l = 0; l < 2; ++l)
     if (ctx->batch_jobs[l]) b += sizeof(BatchJob) * kMaxBatchJobs + sizeof(uint32_t) * (1 + 5 * (kMaxBatchJobs + 1));
   b += ctx->rgb_cap;
+  b += ctx->frame_bytes;
   return b;
 }
 
@@ -937,8 +941,6 @@ rtg_status check_settings(rtg_context* ctx, const char* fn_name, const rtg_rende
     return fail(ctx, RTG_ERR_INVALID_ARG, fn() + ": max_depth must be in [1, 1023]");
   if (cfg->integrator < 0 || cfg->integrator > 2 || cfg->precision < 0 || cfg->precision > 1)
     return fail(ctx, RTG_ERR_INVALID_ARG, fn() + ": bad integrator or precision");
-  if (cfg->integrator == RTG_INTEGRATOR_DIRECT && cfg->max_depth > 31)
-    return fail(ctx, RTG_ERR_UNSUPPORTED, fn() + ": the direct integrator supports max_depth <= 31");
   return RTG_OK;
 }
 
@@ -968,12 +970,44 @@ uint32_t capacity_for(const rtg_context* ctx, uint64_t paths) {
   return std::min(cap, lim);
 }
 
-// The largest chunk (a power of two, at least 1024 paths) whose path state
-// fits `bytes`; 0 if none does.
-uint32_t static_capacity(bool f64, uint64_t bytes) {
-  for (uint32_t cap = 1u << 25; cap >= 1024; cap >>= 1) {
-    const uint64_t need = f64 ? lay_out_paths<double>(nullptr, cap, false, nullptr, nullptr)
-                              : lay_out_paths<float>(nullptr, cap, false, nullptr, nullptr);
+// Direct integrator: bytes of recursion frames per path (max_depth frames;
+// none for the other integrators, whose walks never descend).
+uint64_t frame_bytes_per_path(const rtg_render_settings* cfg) {
+  if (cfg->integrator != RTG_INTEGRATOR_DIRECT) return 0;
+  return (uint64_t)cfg->max_depth * kFrameWords * (cfg->precision == RTG_PRECISION_F64 ? 8 : 4);
+}
+
+// Chunks of a direct render hold at most this much frame memory: 4 Mi FP32
+// paths at depth 12, 64 Ki at depth 1000.
+constexpr uint64_t kFrameBudget = 4ull << 30;
+uint32_t frame_capacity(const rtg_render_settings* cfg, uint32_t cap) {
+  const uint64_t per = frame_bytes_per_path(cfg);
+  if (!per) return cap;
+  while (cap > 1024 && (uint64_t)cap * per > kFrameBudget) cap >>= 1;
+  return cap;
+}
+
+rtg_status ensure_frames(rtg_context* ctx, uint64_t bytes) {
+  if (ctx->frame_bytes >= bytes) return RTG_OK;
+  if (ctx->frame_mem) {
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream2));
+    HIP_TRY(ctx, hipFree(ctx->frame_mem));
+    ctx->frame_mem = nullptr;
+    ctx->frame_bytes = 0;
+  }
+  HIP_TRY(ctx, hipMalloc(&ctx->frame_mem, bytes));
+  ctx->frame_bytes = bytes;
+  return RTG_OK;
+}
+
+// The largest chunk (a power of two, at least one workgroup of paths) whose
+// path state and `per_path` more bytes per path fit `bytes`; 0 if none does.
+uint32_t static_capacity(bool f64, uint64_t bytes, uint64_t per_path) {
+  for (uint32_t cap = 1u << 25; cap >= (uint32_t)kBlock; cap >>= 1) {
+    const uint64_t paths = f64 ? lay_out_paths<double>(nullptr, cap, false, nullptr, nullptr)
+                               : lay_out_paths<float>(nullptr, cap, false, nullptr, nullptr);
+    const uint64_t need = ((paths + 255) & ~uint64_t(255)) + per_path * cap;
     if (need <= bytes) return cap;
   }
   return 0;
@@ -1027,15 +1061,21 @@ rtg_status run_items(rtg_context* ctx, const PassItem* items, uint32_t n, const 
   // rays/s).
   auto cap_of = [&](uint32_t i) {
     const uint64_t paths = paths_of(i);
-    return capacity_for(ctx, lanes_of(i) && paths >= (i + 1 < n ? chunk : split_last) ? (paths + 1) / 2 : paths);
+    return frame_capacity(items[i].cfg, capacity_for(ctx, lanes_of(i) && paths >= (i + 1 < n ? chunk : split_last)
+                                                              ? (paths + 1) / 2
+                                                              : paths));
   };
   uint32_t cap32 = 0, cap64 = 0;
+  uint64_t frames = 0;
   bool two = false;
   for (uint32_t i = 0; i < n; ++i) {
     uint32_t& c = items[i].cfg->precision == RTG_PRECISION_F64 ? cap64 : cap32;
     c = std::max(c, cap_of(i));
+    frames = std::max(frames, frame_bytes_per_path(items[i].cfg) * cap_of(i));
     two = two || (lanes_of(i) && (n > 1 || paths_of(i) >= split_last));
   }
+  if (frames)
+    if (rtg_status st = ensure_frames(ctx, frames)) return st;
   if (cap32) {
     if (rtg_status st = ensure_paths(ctx, 0, cap32, ctx->pb32)) return st;
     if (two)
@@ -1085,6 +1125,8 @@ rtg_status run_items(rtg_context* ctx, const PassItem* items, uint32_t n, const 
         P.stat_slots = kStatSlots;
         P.counts = f64 ? B64.counts : B32.counts;
         P.stats = it.stats;
+        P.frames = ctx->frame_mem;
+        P.frame_stride = cap;
         const hipStream_t st = lane2 ? ctx->stream2 : ctx->stream;
         const hipEvent_t wait = used[lane ^ 1] ? ctx->acc_done[lane ^ 1] : nullptr;
         const hipEvent_t rec = (two || cancel) ? ctx->acc_done[lane] : nullptr;
@@ -1364,14 +1406,20 @@ rtg_status rtg_render_pass_static(rtg_context* ctx, const rtg_scene* scene, cons
   const uint64_t slot_bytes = ((uint64_t)n_slots * 8 + 255) & ~uint64_t(255);
   unsigned long long* slots = reinterpret_cast<unsigned long long*>(ctx->scratch_dev);
   char* base = ctx->scratch_dev + slot_bytes;
-  const uint32_t cap = std::min(capacity_for(ctx, paths), static_capacity(f64, kStaticScratchBytes - slot_bytes));
-  if (cap == 0) return fail(ctx, RTG_ERR_UNSUPPORTED, fn() + ": the scratch array holds no chunk");
+  const uint64_t frame_per_path = frame_bytes_per_path(cfg);
+  const uint32_t cap =
+      std::min(capacity_for(ctx, paths), static_capacity(f64, kStaticScratchBytes - slot_bytes, frame_per_path));
+  if (cap == 0)
+    return fail(ctx, RTG_ERR_UNSUPPORTED,
+                fn() + ": the scratch array holds no chunk of 128 paths at this max_depth (" +
+                    std::to_string(cfg->max_depth) + ")");
   PathBuffers<float> B32{};
   PathBuffers<double> B64{};
   const uint64_t need = f64 ? lay_out_paths<double>(base, cap, false, slots, &B64)
                             : lay_out_paths<float>(base, cap, false, slots, &B32);
-  if (need + slot_bytes > kStaticScratchBytes)
+  if (slot_bytes + ((need + 255) & ~uint64_t(255)) + frame_per_path * cap > kStaticScratchBytes)
     return fail(ctx, RTG_ERR_OUT_OF_MEMORY, fn() + ": chunk state exceeds the scratch array");
+  char* frames = base + ((need + 255) & ~uint64_t(255));
   hipLaunchKernelGGL(k_zero_u64, dim3((n_slots + 127) / 128), dim3(128), 0, ctx->stream, slots, n_slots);
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -1400,6 +1448,8 @@ rtg_status rtg_render_pass_static(rtg_context* ctx, const rtg_scene* scene, cons
       P.stat_slots = kStatSlots;
       P.counts = f64 ? B64.counts : B32.counts;
       P.stats = slots;
+      P.frames = frame_per_path ? frames : nullptr;
+      P.frame_stride = cap;
       const hipEvent_t rec = cancel ? ctx->acc_done[0] : nullptr;
       if (f64)
         run_chunk(scene->dev, B64, P, scene->stack, ctx->stream, side_of(ctx), nullptr, nullptr, rec);

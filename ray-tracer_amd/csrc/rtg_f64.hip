@@ -50,7 +50,7 @@ void chunk_impl(const DevScene& S, const PathBuffers<double>& B, const ChunkPara
                 const SideStream* side, const LaunchHooks* h, hipEvent_t acc_wait, hipEvent_t acc_done) {
   const uint32_t g = grid_of(P.n_paths);
   if (P.integrator != RTG_INTEGRATOR_PATH) {
-    RTG_LAUNCH(h, RTG_K_DIRECT, (k_direct<double, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P);
+    RTG_LAUNCH(h, RTG_K_DIRECT, (k_direct64<STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P);
   } else {
     RTG_LAUNCH(h, RTG_K_GENERATE, k_generate<double>, dim3(grid_of(generate_threads(P))), dim3(kBlock), 0, s, S, B, P);
     const bool nee = P.light_sampling && S.n_lights > 0;

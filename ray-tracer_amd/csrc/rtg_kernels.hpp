@@ -82,6 +82,8 @@ struct ChunkParams {
   uint32_t zero_stats;     // generate clears the job's statistics slots
   uint32_t* counts;        // queue sizes: [b*kSubQueues + s] active, [shadow_counts + b*kSubQueues + s] shadow
   unsigned long long* stats;
+  void* frames;            // direct integrator: the recursion's frames (kFrameWords x max_depth per path)
+  uint32_t frame_stride;   // paths per frame field (the chunk capacity)
 };
 
 // ---------------------------------------------------------------------------
@@ -599,142 +601,149 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(PathBuffers<T> B, ChunkPa
 
 // ---------------------------------------------------------------------------
 // direct / normals integrators: one sample per thread (integrator.cpp:141-198).
+//
+// direct_trace recurses: a metal hit adds albedo x (its reflection's
+// radiance), a dielectric hit tint x F x (reflection) + tint x (1 - F) x
+// (refraction), nested as the reference nests them, so FP64 sums round as
+// the CPU's. The recursion is a depth-first walk over frames. The frame being
+// worked on lives in registers; the frames of the enclosing calls wait in
+// HBM, one field per row of `frame_stride` lanes (the paths of a chunk side
+// by side, so a wave's pushes and pops coalesce), max_depth frames per path:
+// any depth the scene format allows (1000) is followed.
+constexpr int kFrameWords = 16;  // L, w1, w2, o2, d2 (3 each) and phase | second << 2
+enum : int { kFrL = 0, kFrW1 = 3, kFrW2 = 6, kFrO2 = 9, kFrD2 = 12, kFrFlags = 15 };
+
+template <class T>
+struct FrameStack {
+  T* base;
+  uint32_t stride, lane;
+  __device__ T& at(int frame, int field) const {
+    return base[((size_t)frame * kFrameWords + (size_t)field) * stride + lane];
+  }
+  __device__ void put(int frame, int field, const V3<T>& v) const {
+    at(frame, field) = v.x;
+    at(frame, field + 1) = v.y;
+    at(frame, field + 2) = v.z;
+  }
+  __device__ V3<T> get(int frame, int field) const {
+    return V3<T>{at(frame, field), at(frame, field + 1), at(frame, field + 2)};
+  }
+};
+
 template <class T, int STACK>
-__device__ V3<T> direct_sample(const DevScene& S, const ChunkParams& P, V3<T> o, V3<T> d,
+__device__ V3<T> direct_sample(const DevScene& S, const ChunkParams& P, uint32_t lane, V3<T> o, V3<T> d,
                                typename ModeOps<T>::R& rng, LdsStack stk, uint32_t* st) {
-  struct Frame {
-    V3<T> L, w1, w2, o2, d2;
-    int phase;  // 0 trace, 1 waiting first child, 2 waiting second child
-    bool second;
-  };
-  constexpr int kMaxFrames = 33;
-  Frame fr[kMaxFrames];
+  const FrameStack<T> fs{static_cast<T*>(P.frames), P.frame_stride, lane};
+  // Two modes. Tracing: (a, b) is the ray of the frame at depth `top`.
+  // Returning: a is the radiance of a finished call, handed to the frame
+  // at `top` (its caller; none when top < 0). Only these few values are
+  // carried from step to step; a frame's weights and pending second ray wait
+  // in HBM.
+  bool tracing = true;
   int top = 0;
-  fr[0].phase = 0;
-  fr[0].o2 = o;
-  fr[0].d2 = d;
-  V3<T> ret{0, 0, 0};
-  bool have_ret = false;
-  while (top >= 0) {
-    Frame& f = fr[top];
-    if (have_ret) {
-      have_ret = false;
-      if (f.phase == 1) {
-        f.L = f.L + f.w1 * ret;
-        if (f.second) {
-          f.phase = 2;
-          if (top + 1 >= kMaxFrames) { ret = f.L; have_ret = true; --top; continue; }
-          fr[top + 1].phase = 0;
-          fr[top + 1].o2 = f.o2;
-          fr[top + 1].d2 = f.d2;
+  V3<T> a = o, b = d;
+  while (true) {
+    if (!tracing) {
+      if (top < 0) return a;
+      const int flags = (int)fs.at(top, kFrFlags);
+      V3<T> L = fs.get(top, kFrL);
+      if ((flags & 3) == 1) {
+        L = L + fs.get(top, kFrW1) * a;
+        if (flags & 4) {  // then the refracted ray
+          fs.put(top, kFrL, L);
+          fs.at(top, kFrFlags) = T(2);
+          a = fs.get(top, kFrO2);
+          b = fs.get(top, kFrD2);
           ++top;
+          tracing = true;
           continue;
         }
       } else {
-        f.L = f.L + f.w2 * ret;
+        L = L + fs.get(top, kFrW2) * a;
       }
-      ret = f.L;
-      have_ret = true;
+      a = L;
       --top;
       continue;
     }
-    // phase 0: trace fr[top].o2/d2 at depth `top`
-    const V3<T> ro = f.o2, rd = f.d2;
+    const V3<T> ro = a, rd = b;
     Counters c{0, 0};
     ++st[kStRays];
     const auto h = ModeOps<T>::closest(S, P.use_bvh != 0, ro, rd, stk, c);
     st[kStVisits] += c.visits;
     st[kStTests] += c.tests;
+    tracing = false;  // unless a child ray follows
     if (h.prim == 0xFFFFFFFFu && h.plane < 0) {
       // Normals: a miss is black (integrator.cpp:190); direct: the sky.
-      ret = P.integrator == RTG_INTEGRATOR_NORMALS ? V3<T>{0, 0, 0} : background<T>(S, rd);
-      have_ret = true;
+      a = P.integrator == RTG_INTEGRATOR_NORMALS ? V3<T>{0, 0, 0} : background<T>(S, rd);
       --top;
       continue;
     }
     HitRec<T> rec;
     ModeOps<T>::finalize(S, ro, rd, h.t, h.b1, h.b2, h.prim, h.plane, rec);
     const Mat<T> mat = load_mat<T>(S, rec.material);
-    f.L = emitted(mat, rec.front_face);
+    V3<T> L = emitted(mat, rec.front_face);
     if (P.integrator == RTG_INTEGRATOR_NORMALS) {
       const V3<T> nn = rec.front_face ? rec.normal : -rec.normal;
-      ret = (nn + V3<T>{1, 1, 1}) * T(0.5);
-      have_ret = true;
+      a = (nn + V3<T>{1, 1, 1}) * T(0.5);
       --top;
       continue;
     }
-    if (top >= P.max_depth) {
-      ret = f.L;
-      have_ret = true;
-      --top;
-      continue;
-    }
-    if (mat.kind == RTG_MAT_LAMBERTIAN) {
-      for (uint32_t li = 0; li < S.n_lights; ++li) {
-        ShadowReq<T> req;
-        if (ModeOps<T>::light(S, li, rec, mat, rng, req)) {
-          ++st[kStShadow];
-          Counters c2{0, 0};
-          if (!ModeOps<T>::occluded(S, P.use_bvh != 0, req.o, req.d, req.tmax, stk, c2)) f.L = f.L + req.f_rop;
-          st[kStSVisits] += c2.visits;
-          st[kStSTests] += c2.tests;
+    if (top < P.max_depth) {
+      if (mat.kind == RTG_MAT_LAMBERTIAN) {
+        for (uint32_t li = 0; li < S.n_lights; ++li) {
+          ShadowReq<T> req;
+          if (ModeOps<T>::light(S, li, rec, mat, rng, req)) {
+            ++st[kStShadow];
+            Counters c2{0, 0};
+            if (!ModeOps<T>::occluded(S, P.use_bvh != 0, req.o, req.d, req.tmax, stk, c2)) L = L + req.f_rop;
+            st[kStSVisits] += c2.visits;
+            st[kStSTests] += c2.tests;
+          }
         }
-      }
-      ret = f.L;
-      have_ret = true;
-      --top;
-      continue;
-    }
-    if (mat.kind == RTG_MAT_METAL) {
-      Scatter<T> sr;
-      if (scatter(mat, rd, rec, rng, sr) && top + 1 < kMaxFrames) {
-        f.w1 = mat.c0;
-        f.second = false;
-        f.phase = 1;
-        fr[top + 1].phase = 0;
-        fr[top + 1].o2 = offset_ray_origin(rec.p, rec.gn, sr.dir);
-        fr[top + 1].d2 = sr.dir;
+      } else if (mat.kind == RTG_MAT_METAL) {
+        Scatter<T> sr;
+        if (scatter(mat, rd, rec, rng, sr)) {
+          fs.put(top, kFrL, L);
+          fs.put(top, kFrW1, mat.c0);
+          fs.at(top, kFrFlags) = T(1);
+          a = offset_ray_origin(rec.p, rec.gn, sr.dir);
+          b = sr.dir;
+          ++top;
+          tracing = true;
+          continue;
+        }
+      } else if (mat.kind == RTG_MAT_DIELECTRIC) {
+        const V3<T> in = normalize(rd);
+        const T eta = rec.front_face ? T(1) / mat.p0 : mat.p0;
+        const T cos_i = r_fmin(dot(-in, rec.normal), T(1));
+        V3<T> refracted;
+        const bool can = refract(in, rec.normal, eta, &refracted);
+        const T F = can ? schlick(cos_i, eta) : T(1);
+        const V3<T> rdir = reflect(in, rec.normal);
+        fs.put(top, kFrL, L);
+        fs.put(top, kFrW1, mat.c0 * F);
+        fs.at(top, kFrFlags) = T(can ? 5 : 1);
+        if (can) {
+          const V3<T> tdir = normalize(refracted);
+          fs.put(top, kFrW2, mat.c0 * (T(1) - F));
+          fs.put(top, kFrO2, offset_ray_origin(rec.p, rec.gn, tdir));
+          fs.put(top, kFrD2, tdir);
+        }
+        a = offset_ray_origin(rec.p, rec.gn, rdir);
+        b = rdir;
         ++top;
+        tracing = true;
         continue;
       }
-      ret = f.L;
-      have_ret = true;
-      --top;
-      continue;
     }
-    if (mat.kind == RTG_MAT_DIELECTRIC && top + 1 < kMaxFrames) {
-      const V3<T> in = normalize(rd);
-      const T eta = rec.front_face ? T(1) / mat.p0 : mat.p0;
-      const T cos_i = r_fmin(dot(-in, rec.normal), T(1));
-      V3<T> refracted;
-      const bool can = refract(in, rec.normal, eta, &refracted);
-      const T F = can ? schlick(cos_i, eta) : T(1);
-      const V3<T> rdir = reflect(in, rec.normal);
-      f.w1 = mat.c0 * F;
-      f.second = can;
-      if (can) {
-        const V3<T> tdir = normalize(refracted);
-        f.w2 = mat.c0 * (T(1) - F);
-        f.o2 = offset_ray_origin(rec.p, rec.gn, tdir);
-        f.d2 = tdir;
-      }
-      f.phase = 1;
-      fr[top + 1].phase = 0;
-      fr[top + 1].o2 = offset_ray_origin(rec.p, rec.gn, rdir);
-      fr[top + 1].d2 = rdir;
-      ++top;
-      continue;
-    }
-    ret = f.L;
-    have_ret = true;
+    a = L;
     --top;
   }
-  return ret;
 }
 
 template <class T, int STACK>
-__global__ __launch_bounds__(kBlock) void k_direct(DevScene S, PathBuffers<T> B, ChunkParams P) {
-  __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
+__device__ __forceinline__ void direct_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P,
+                                            uint32_t* sref) {
   const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t pid = P.pid_base + l;
   uint32_t st[kStCount] = {0};
@@ -748,7 +757,7 @@ __global__ __launch_bounds__(kBlock) void k_direct(DevScene S, PathBuffers<T> B,
       const double jx = rng.next();
       const double jy = rng.next();
       ModeOps<T>::camera(S, x, y, jx, jy, rng, o, d);
-      L = direct_sample<T, STACK>(S, P, o, d, rng, lds_stack(sref), st);
+      L = direct_sample<T, STACK>(S, P, l, o, d, rng, lds_stack(sref), st);
     } else {
       HashDraws rng(path_key(P.seed, pixel, sample), 0xFFFFFFFFu);
       const float jx = rng.next();
@@ -756,13 +765,27 @@ __global__ __launch_bounds__(kBlock) void k_direct(DevScene S, PathBuffers<T> B,
       ModeOps<T>::camera(S, x, y, jx, jy, rng, o, d);
       rng.bounce = 0;
       rng.idx = 0;
-      L = direct_sample<T, STACK>(S, P, o, d, rng, lds_stack(sref), st);
+      L = direct_sample<T, STACK>(S, P, l, o, d, rng, lds_stack(sref), st);
     }
     B.lx[pid] = L.x;
     B.ly[pid] = L.y;
     B.lz[pid] = L.z;
   }
   flush_stats(P, st, blockIdx.x);
+}
+// FP32 at 4 waves per SIMD (128 VGPRs, a 60-byte spill; 134 VGPRs and 3
+// waves otherwise); FP64 as the compiler allocates it (217 VGPRs, 2 waves).
+template <int STACK>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(STACK <= 31 ? 4 : 1))) void k_direct32(DevScene S,
+                                                                                        PathBuffers<float> B,
+                                                                                        ChunkParams P) {
+  __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
+  direct_body<float, STACK>(S, B, P, sref);
+}
+template <int STACK>
+__global__ __launch_bounds__(kBlock) void k_direct64(DevScene S, PathBuffers<double> B, ChunkParams P) {
+  __shared__ uint32_t sref[(STACK > 0 ? STACK : 1) * kBlock];
+  direct_body<double, STACK>(S, B, P, sref);
 }
 
 // ---------------------------------------------------------------------------

@@ -5,7 +5,9 @@ oracle/Makefile). Run in the container that has the reference checkout:
     make -C oracle && python tests/golden/make_golden.py
 
 `--messages` rewrites only the scene-format fixtures (validation messages of
-bad_scenes.json, dumps of dump_scenes.json) in reference_meta.json.
+bad_scenes.json, dumps of dump_scenes.json) in reference_meta.json;
+`--direct-deep` only reference_direct_deep.npz (the direct integrator far
+past 31 bounces).
 Each fixture is data only: inputs and the reference's outputs.
 """
 import ctypes as C
@@ -58,6 +60,34 @@ def scene_format(L, meta):
         L.ref_dump_scene(text.encode(), buf, len(buf))
         dumps.append(buf.value.decode())
     meta["dump_scene_texts"] = dumps
+
+
+# The direct integrator recursing deep: a corridor of two facing mirrors
+# (tests/golden/mirror_corridor.json) at max_depth 1000 and 200, and glass
+# and metal (two-way splits) at max_depth 40.
+DIRECT_DEEP = [("mirror_corridor.json", 12, 8, 2, 1000), ("mirror_corridor.json", 12, 8, 2, 200),
+               ("cornell_glass_metal.json", 10, 10, 2, 40)]
+
+
+def deep_scene(name):
+    if name == "mirror_corridor.json":
+        with open(os.path.join(HERE, name)) as f:
+            return json.load(f)
+    return scenes.load_asset(name)
+
+
+def direct_deep():
+    assert orc.have_ref(), "oracle/_ref/librtref.so missing: run make -C oracle in the reference container"
+    out = {}
+    for name, w, h_, spp, depth in DIRECT_DEEP:
+        r = orc.RefScene(json.dumps(deep_scene(name)), width=w, height=h_, integrator="direct", max_depth=depth)
+        s, c, st = r.render_rows(0, spp, w, h_)
+        key = f"{name[:-5]}_{depth}"
+        out[key + "_sum"], out[key + "_count"] = s, c
+        out[key + "_stats"] = np.array([int(x) for x in st], np.uint64)
+    path = os.path.join(HERE, "reference_direct_deep.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path)
 
 
 def messages_only():
@@ -226,4 +256,9 @@ def main():
 
 
 if __name__ == "__main__":
-    messages_only() if "--messages" in sys.argv[1:] else main()
+    if "--messages" in sys.argv[1:]:
+        messages_only()
+    elif "--direct-deep" in sys.argv[1:]:
+        direct_deep()
+    else:
+        main()

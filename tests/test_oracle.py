@@ -105,6 +105,28 @@ def test_framebuffer_sums_match_reference(key, name, integ, asset):
     assert [st["camera_samples"], st["rays"], st["shadow_rays"], st["rr_terminated"], st["invalid_samples"]] == m["stats"]
 
 
+DEEP = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_direct_deep.npz"))
+
+
+@pytest.mark.parametrize("name,w,h,spp,depth", [("mirror_corridor.json", 12, 8, 2, 1000),
+                                                ("mirror_corridor.json", 12, 8, 2, 200),
+                                                ("cornell_glass_metal.json", 10, 10, 2, 40)])
+def test_deep_direct_matches_reference(name, w, h, spp, depth):
+    """direct_trace far past 31 bounces (max_depth up to 1000)."""
+    if name == "mirror_corridor.json":
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)) as f:
+            j = json.load(f)
+    else:
+        j = scenes.load_asset(name)
+    key = f"{name[:-5]}_{depth}"
+    o = orc.OracleScene(j, width=w, height=h, integrator="direct", max_depth=depth)
+    s, c, st = o.render(0, spp, threads=4)
+    assert np.array_equal(c, DEEP[key + "_count"])
+    assert np.array_equal(s.view(np.uint64), DEEP[key + "_sum"].view(np.uint64))
+    assert [st["camera_samples"], st["rays"], st["shadow_rays"], st["rr_terminated"], st["invalid_samples"]] == \
+        [int(x) for x in DEEP[key + "_stats"]]
+
+
 def test_camera_rays_match_reference():
     cam = GOLD["camera_desc"]
     camera = rtg.Camera()
