@@ -1150,17 +1150,34 @@ rtg_status run_items(rtg_context* ctx, const PassItem* items, uint32_t n, const 
   return cancelled ? RTG_ERR_CANCELLED : RTG_OK;
 }
 
+// Unobserved passes of one call merged into one item unless RTG_MERGE_PASSES=0.
+bool merge_passes_enabled() {
+  const char* e = std::getenv("RTG_MERGE_PASSES");
+  return !(e && e[0] == '0');
+}
+
 // Passes [bounds[i], bounds[i + 1]) of pixels [px0, px1), i < n_pass, of one
 // render (run_items: passes side by side, bitwise the passes one by one).
 rtg_status render_ranges(rtg_context* ctx, const rtg_scene* scene, const rtg_render_settings* cfg,
                          const uint32_t* bounds, uint32_t n_pass, uint32_t px0, uint32_t px1, rtg_accum* acc,
                          const volatile int32_t* cancel, uint32_t* passes_done, rtg_stats* stats, const char* fn) {
-  std::vector<PassItem> items(n_pass);
-  for (uint32_t i = 0; i < n_pass; ++i) items[i] = PassItem{scene, cfg, acc, bounds[i], bounds[i + 1], px0, px1, ctx->stats};
+  // Without a cancel flag nobody sees the passes apart: they run as one item
+  // of all their samples, cut into full chunks (every pixel still adds its
+  // samples in sample order, so the sums and counters are bitwise those of
+  // the passes one by one). Small early passes no longer leave the GPU half
+  // idle in their late bounces. With a cancel flag the passes stay items of
+  // their own, so a cancelled call reports the passes it completed.
+  const bool merge = cancel == nullptr && merge_passes_enabled();
+  std::vector<PassItem> items;
+  if (merge) items.push_back(PassItem{scene, cfg, acc, bounds[0], bounds[n_pass], px0, px1, ctx->stats});
+  else
+    for (uint32_t i = 0; i < n_pass; ++i)
+      items.push_back(PassItem{scene, cfg, acc, bounds[i], bounds[i + 1], px0, px1, ctx->stats});
   HIP_TRY(ctx, hipMemsetAsync(ctx->stats, 0, sizeof(unsigned long long) * kStatSlots * kStatStride, ctx->stream));
   HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   uint32_t done = 0;
-  const rtg_status rs = run_items(ctx, items.data(), n_pass, cancel, &done);
+  const rtg_status rs = run_items(ctx, items.data(), (uint32_t)items.size(), cancel, &done);
+  if (merge && rs == RTG_OK) done = n_pass;
   if (rs != RTG_OK && rs != RTG_ERR_CANCELLED) return rs;
   if (rtg_status st = stage_stats(ctx)) return st;
   HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));

@@ -1,5 +1,6 @@
 """rtg_render_passes: a progressive schedule in one call, the passes running
-side by side on the device. Every pixel's additions must stay in sample
+side by side on the device (or, with no cancel flag, merged into one item of
+all their samples). Every pixel's additions must stay in sample
 order, so the framebuffer (sums and counts) and the counters must be bitwise
 those of rtg_render_pass called once per pass — in both precisions, for the
 direct integrator (one lane), with one lane forced, and with passes cut into
@@ -54,12 +55,16 @@ def test_passes_in_one_call_equal_pass_by_pass(ctx, monkeypatch, name, w, h, spp
     bounds = schedule(spp)
     s0, c0, k0 = one_by_one(ctx, dev, cfg, bounds)
     assert int(c0.min()) == spp and k0["rays"] > 0
-    for lanes, chunk in (("2", None), ("1", None), ("2", "2048")):
+    # merged: the passes run as one item of all their samples (no cancel
+    # flag); "0": each pass an item of its own
+    for lanes, chunk, merge in (("2", None, "1"), ("2", None, "0"), ("1", None, "1"), ("2", "2048", "1"),
+                                ("2", "2048", "0")):
         monkeypatch.setenv("RTG_LANES", lanes)
+        monkeypatch.setenv("RTG_MERGE_PASSES", merge)
         if chunk:
             monkeypatch.setenv("RTG_CHUNK_PATHS", chunk)
         s1, c1, k1 = in_one_call(ctx, dev, cfg, bounds)
-        case = (lanes, chunk)
+        case = (lanes, chunk, merge)
         assert np.array_equal(c1, c0), case
         assert np.array_equal(s1.view(np.uint64), s0.view(np.uint64)), case
         assert k1 == k0, case
