@@ -223,6 +223,18 @@ def batch_leg(rtg, seed=4, count=64):
     # The autotuner picks the scheduling knobs (queue / chunk capacity, wide
     # batch passes, lanes) by timing this workload's calls, then the batched
     # workload is timed again under its choice.
+    # Every render whole (all its samples) as one job of a single batch call:
+    # the 64 renders in one launch sequence; each pixel still adds its
+    # samples in order, so the images equal the pass-by-pass ones.
+    for a in accs:
+        a.clear()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    rays_o = sum(st.rays + st.shadow_rays for st in
+                 ctx.render_batch([(devs[i], accs[i], cfgs[i], 0, scheds[i][-1][1]) for i in range(count)]))
+    ctx.synchronize()
+    one_call_s = time.perf_counter() - t0
+    identical = identical and digests() == ref and rays_o == rays_b
     pass_calls = [[(devs[i], accs[i], cfgs[i]) + scheds[i][p] for i in range(count) if p < len(scheds[i])]
                   for p in range(max(len(s) for s in scheds))]
     t0 = time.perf_counter()
@@ -253,6 +265,8 @@ def batch_leg(rtg, seed=4, count=64):
             "plain_g_rays_per_s": rays_p / plain_s / 1e9, "plain_passes_call_g_rays_per_s": rays_q / passes_s / 1e9,
             "batched_g_rays_per_s": rays_b / batched_s / 1e9,
             "speedup": plain_s / batched_s, "speedup_vs_passes_call": passes_s / batched_s,
+            "batched_one_call_s": round(one_call_s, 4), "batched_one_call_g_rays_per_s": rays_b / one_call_s / 1e9,
+            "speedup_one_call": plain_s / one_call_s, "speedup_one_call_vs_passes_call": passes_s / one_call_s,
             "autotune": tune, "batched_tuned_s": round(tuned_s, 4),
             "batched_tuned_g_rays_per_s": rays_b / tuned_s / 1e9,
             "speedup_tuned": plain_s / tuned_s, "speedup_tuned_vs_passes_call": passes_s / tuned_s,

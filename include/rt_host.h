@@ -145,7 +145,7 @@ typedef struct rts_stats {
 } rts_stats;
 
 /* budget_bytes 0: 64 GiB; tick_paths: final-render paths per tick, 0 = tuned
- * (each tick aimed at 7 ms with previews in it, 20 ms without, from the
+ * (each tick aimed at 7 ms with previews in it, 16 ms without, from the
  * scenes' measured cost per path). */
 int rts_create(int32_t device, uint64_t budget_bytes, uint32_t tick_paths, rts_server** out);
 /* Replaces the footprint budget (0: 64 GiB) for the next rts_run. */

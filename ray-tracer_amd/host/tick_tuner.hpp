@@ -26,7 +26,7 @@ constexpr double kTickTargetMs = 7.0;
 // at most this long before its own tick, still inside 50 ms, and the finals
 // lose less to tick boundaries (each one a host round trip with the GPU
 // idle).
-constexpr double kFinalsTickTargetMs = 20.0;
+constexpr double kFinalsTickTargetMs = 16.0;
 
 class TickTuner {
  public:
