@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # Algorithmic bytes of the FP32 closest-hit kernel: 32 B per node box tested
 # (a 64-byte record holds both children), 48 B per triangle test, 28 B of ray
 # read (queue slot + origin + direction) and 20 B of hit written per ray.
-NODE_B, TRI_B, RAY_IN_B, HIT_OUT_B = 32, 48, 28, 20
+NODE_B, TRI_B, RAY_IN_B, HIT_OUT_B, RADIANCE_OUT_B = 32, 48, 28, 20, 12
 
 
 def load_pkg():
@@ -124,21 +124,32 @@ def cpu_baseline(scene_text, width, height, spp_full, target_s=12.0, single_s=4.
                               "sample": f"{reps_1} x {w1}x{h1} at {spp_1} spp, {rays_1} rays in {secs_1:.2f} s"}}
 
 
-def extend_roofline(prof, traffic):
-    """Roofline of the closest-hit kernel from a profile: algorithmic bytes per
-    launch (node boxes, triangle tests, ray in + hit out) over its average
-    launch duration (HIP events recorded by each launch's dispatch)."""
-    ext = prof["extend"]
+def extend_roofline(prof, traffic, kind="extend"):
+    """Roofline of the traversal kernel from a profile: algorithmic bytes per
+    launch over its average launch duration (HIP events recorded by each
+    launch's dispatch). kind "extend": the path integrator's closest-hit
+    kernel (node boxes, primitive tests, ray in + hit out); kind "direct": the
+    direct / normals kernel, which traces a sample's closest-hit and shadow
+    rays itself with rays and hits in registers (node boxes and primitive
+    tests of both, 12 B of radiance out per sample; its recursion frames are
+    not counted, so this is a lower bound)."""
+    ext = prof[kind]
     n_launch = max(1, ext["launches"])
-    bytes_total = (NODE_B * prof["closest_visits"] + TRI_B * prof["closest_tests"] +
-                   (RAY_IN_B + HIT_OUT_B) * prof["closest_rays"])
+    if kind == "direct":
+        bytes_total = (NODE_B * (prof["closest_visits"] + prof["shadow_visits"]) +
+                       TRI_B * (prof["closest_tests"] + prof["shadow_tests"]) + RADIANCE_OUT_B * prof["samples"])
+    else:
+        bytes_total = (NODE_B * prof["closest_visits"] + TRI_B * prof["closest_tests"] +
+                       (RAY_IN_B + HIT_OUT_B) * prof["closest_rays"])
     per_launch = bytes_total / n_launch
     avg_ms = ext["ms"] / n_launch
     achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     # What the HBM itself carries per launch (FETCH_SIZE + WRITE_SIZE counters,
     # scaled by a copy of known size) over the same launch time.
     hbm = traffic / (avg_ms * 1e-3) / 1e9 if traffic and avg_ms > 0 else None
-    return {"kernel": "k_extend (FP32 closest-hit traversal)", "bound": "hbm", "achieved": achieved,
+    label = ("k_direct (FP32 direct / normals sample, closest-hit and shadow traversal)" if kind == "direct"
+             else "k_extend (FP32 closest-hit traversal)")
+    return {"kernel": label, "bound": "hbm", "achieved": achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "hbm_achieved": hbm, "hbm_frac": hbm / HBM_PEAK_GBS if hbm is not None else None,
             "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches": ext["launches"],
@@ -220,9 +231,6 @@ def batch_leg(rtg, seed=4, count=64):
     ctx.synchronize()
     batched_s = time.perf_counter() - t0
     identical = digests() == ref
-    # The autotuner picks the scheduling knobs (queue / chunk capacity, wide
-    # batch passes, lanes) by timing this workload's calls, then the batched
-    # workload is timed again under its choice.
     # Every render whole (all its samples) as one job of a single batch call:
     # the 64 renders in one launch sequence; each pixel still adds its
     # samples in order, so the images equal the pass-by-pass ones.
@@ -237,6 +245,9 @@ def batch_leg(rtg, seed=4, count=64):
     identical = identical and digests() == ref and rays_o == rays_b
     pass_calls = [[(devs[i], accs[i], cfgs[i]) + scheds[i][p] for i in range(count) if p < len(scheds[i])]
                   for p in range(max(len(s) for s in scheds))]
+    # The autotuner picks the scheduling knobs (queue / chunk capacity, wide
+    # batch passes, lanes) by timing this workload's calls, then the batched
+    # workload is timed again under its choice.
     t0 = time.perf_counter()
     tune = ctx.autotune(pass_calls, key=f"batch-seed{seed}-{count}")
     tune["tuning_s"] = round(time.perf_counter() - t0, 2)
@@ -424,7 +435,8 @@ def main():
     ctx.synchronize()
     # Only the closest-hit launches are timed in the timed region (the
     # roofline's kernel): each timed launch perturbs the pass a little.
-    rtg.set_profiling(ctx, True, kinds=None if args.profile_all else ["extend"])
+    kind = "direct" if cfg.integrator != rtg.INTEGRATORS["path"] else "extend"
+    rtg.set_profiling(ctx, True, kinds=None if args.profile_all else [kind])
     ctx.synchronize()
     t0 = time.perf_counter()
     rays = 0
@@ -433,17 +445,20 @@ def main():
     ctx.synchronize()
     secs = time.perf_counter() - t0
     prof = rtg.get_profile(ctx).as_dict()
+    prof["samples"] = cfg.width * cfg.height * spp * args.steps
     rtg.set_profiling(ctx, False)
     value = rays / secs / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_extend_{args.workload}.json")
-    if os.path.exists(tpath):
+    if kind == "extend" and os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-    roofline = extend_roofline(prof, traffic)
+    roofline = extend_roofline(prof, traffic, kind)
     node_bytes = NODE_B * (prof["closest_visits"] + prof["shadow_visits"])
     node_rate = node_bytes / secs / 1e9
-    roofline["concurrent"] = ("the pass schedule runs in one call, its chunks alternating between two streams "
+    roofline["concurrent"] = ("direct / normals renders run on one stream; the kernel is alone on the GPU"
+                              if kind == "direct" else
+                              "the pass schedule runs in one call, its chunks alternating between two streams "
                               "(consecutive passes side by side), each with its any-hit kernels on a third or "
                               "fourth stream, so a launch shares the GPU with up to three other kernels and its "
                               "duration includes that time; roofline_serial is the kernel alone")
@@ -458,7 +473,8 @@ def main():
         render_once(rtg, ctx, dev, acc, cfg, spp)
     ctx.synchronize()
     sprof = rtg.get_profile(ctx).as_dict()
-    roofline_serial = extend_roofline(sprof, None)  # the traffic file holds the timed render's launches
+    sprof["samples"] = cfg.width * cfg.height * spp * 2
+    roofline_serial = extend_roofline(sprof, None, kind)  # the traffic file holds the timed render's launches
     roofline_serial["kernel_ms_per_render"] = {k: round(v["ms"] / 2, 3) for k, v in sprof.items()
                                                if isinstance(v, dict) and v["launches"]}
     rtg.set_profiling(ctx, False)
