@@ -435,7 +435,11 @@ def main():
     ctx.synchronize()
     # Only the closest-hit launches are timed in the timed region (the
     # roofline's kernel): each timed launch perturbs the pass a little.
-    kind = "direct" if cfg.integrator != rtg.INTEGRATORS["path"] else "extend"
+    # FP32 direct runs the weighted ray queue, whose closest-hit kernel
+    # (k_dextend) is timed like the path integrator's; FP64 direct and the
+    # normals integrator run one kernel per sample.
+    queue = cfg.integrator == rtg.INTEGRATORS["direct"] and precision == rtg.PRECISION_F32
+    kind = "direct" if cfg.integrator != rtg.INTEGRATORS["path"] and not queue else "extend"
     rtg.set_profiling(ctx, True, kinds=None if args.profile_all else [kind])
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -454,10 +458,12 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = extend_roofline(prof, traffic, kind)
+    if queue:
+        roofline["kernel"] = "k_dextend (FP32 direct: closest hit of one generation of the weighted ray queue)"
     node_bytes = NODE_B * (prof["closest_visits"] + prof["shadow_visits"])
     node_rate = node_bytes / secs / 1e9
     roofline["concurrent"] = ("direct / normals renders run on one stream; the kernel is alone on the GPU"
-                              if kind == "direct" else
+                              if kind == "direct" or queue else
                               "the pass schedule runs in one call, its chunks alternating between two streams "
                               "(consecutive passes side by side), each with its any-hit kernels on a third or "
                               "fourth stream, so a launch shares the GPU with up to three other kernels and its "
@@ -475,6 +481,7 @@ def main():
     sprof = rtg.get_profile(ctx).as_dict()
     sprof["samples"] = cfg.width * cfg.height * spp * 2
     roofline_serial = extend_roofline(sprof, None, kind)  # the traffic file holds the timed render's launches
+    roofline_serial["kernel"] = roofline["kernel"]
     roofline_serial["kernel_ms_per_render"] = {k: round(v["ms"] / 2, 3) for k, v in sprof.items()
                                                if isinstance(v, dict) and v["launches"]}
     rtg.set_profiling(ctx, False)

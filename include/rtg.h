@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define RTG_ABI_VERSION 1
+#define RTG_ABI_VERSION 2
 
 typedef int32_t rtg_status;
 enum {
@@ -154,6 +154,8 @@ typedef struct rtg_stats {
   uint64_t node_visits;   /* BVH node box tests */
   uint64_t prim_tests;    /* primitive intersection tests */
   double elapsed_ms;      /* device time of the call (HIP events) */
+  uint64_t redone_chunks; /* FP32 direct: chunks whose ray queue filled, recomputed
+                             depth first (same result; a measure of wasted work) */
 } rtg_stats;
 
 /* ------------------------------------------------------------------------ */

@@ -821,7 +821,8 @@ uint64_t lay_out_paths(char* base, uint32_t cap, bool batch, unsigned long long*
   // queue (10 T + sq_pid) and 3 shadow-radiance arrays.
   constexpr bool f32 = sizeof(T) == 4;
   const uint64_t sq = f32 ? 2 * c : c;  // shadow-queue entries
-  const uint64_t n_counts = batch ? (uint64_t)kMaxBatchJobs * kBatchCounts : 2ull * kMaxBounce * kSubQueues;
+  const uint64_t n_counts =
+      (batch ? (uint64_t)kMaxBatchJobs * kBatchCounts : 2ull * kMaxBounce * kSubQueues) * kCountStride;
   const uint64_t n_stats = batch ? (uint64_t)kMaxBatchJobs * kBatchStatSlots * kStatStride : 0;
   const uint64_t bytes = al(c * sizeof(T)) * (15 + (f32 ? 3 : 0)) + al(sq * sizeof(T)) * 10 + al(c * 8) * 2 +
                          al(c * 4) * 4 + al(sq * 4) + al(c) + al(sizeof(uint32_t) * n_counts) + al(8 * n_stats);
@@ -908,6 +909,7 @@ void add_counters(rtg_stats* out, const unsigned long long* h) {
   out->invalid_samples += h[kStInvalid];
   out->node_visits += h[kStVisits] + h[kStSVisits];
   out->prim_tests += h[kStTests] + h[kStSTests];
+  out->redone_chunks += h[kStRedone];
 }
 
 // Enqueues the copy of the statistics slots into pinned memory (before the
@@ -970,20 +972,46 @@ uint32_t capacity_for(const rtg_context* ctx, uint64_t paths) {
   return std::min(cap, lim);
 }
 
-// Direct integrator: bytes of recursion frames per path (max_depth frames;
-// none for the other integrators, whose walks never descend).
-uint64_t frame_bytes_per_path(const rtg_render_settings* cfg) {
+// Direct integrator memory of a chunk: FP64 keeps one sample's recursion
+// frames per path (max_depth frames); FP32 runs the weighted ray queue
+// (rtg_direct.hpp). None for the other integrators, whose walks never
+// descend.
+bool direct_forced(const rtg_render_settings* cfg, uint32_t n_lights) {
+  const char* e = std::getenv("RTG_DIRECT");
+  if (e && std::strcmp(e, "dfs") == 0) return true;
+  return cfg->max_depth > kDirectWaveDepth || n_lights > kDirectWaveLights;
+}
+// Ray slots of one FP32 direct generation: twice the chunk's samples, and
+// at least two per thread of a block in every sub-queue (a block appends to
+// one sub-queue, so small chunks would otherwise overflow at the first
+// generation). RTG_DIRECT_GROWTH sets the factor alone (tests use tiny
+// queues); a generation that outgrows its slots makes its chunk run depth
+// first, with the same result.
+uint32_t direct_entries(uint32_t cap) {
+  double f = 2.0, lo = 2.0 * kBlock * kSubQueues;
+  if (const char* e = std::getenv("RTG_DIRECT_GROWTH")) {
+    f = std::atof(e);
+    lo = kSubQueues;
+  }
+  return (uint32_t)std::clamp(f * cap, lo, (double)(1u << 30));
+}
+uint64_t chunk_extra_bytes(const rtg_render_settings* cfg, uint32_t n_lights, uint32_t cap) {
   if (cfg->integrator != RTG_INTEGRATOR_DIRECT) return 0;
-  return (uint64_t)cfg->max_depth * kFrameWords * (cfg->precision == RTG_PRECISION_F64 ? 8 : 4);
+  if (cfg->precision == RTG_PRECISION_F64) return (uint64_t)cfg->max_depth * kFrameWords * 8 * cap;
+  return direct_layout(nullptr, cap, direct_entries(cap), n_lights, cfg->max_depth, direct_forced(cfg, n_lights),
+                       nullptr);
+}
+void set_direct(ChunkParams& P, const rtg_render_settings* cfg, uint32_t n_lights, uint32_t cap) {
+  P.frame_stride = cap;
+  P.dq_entries = direct_entries(cap);
+  P.dq_forced = direct_forced(cfg, n_lights) ? 1u : 0u;
 }
 
-// Chunks of a direct render hold at most this much frame memory: 4 Mi FP32
-// paths at depth 12, 64 Ki at depth 1000.
+// Chunks of a direct render hold at most this much integrator memory: 4 Mi
+// FP64 paths at depth 12, 64 Ki at depth 1000.
 constexpr uint64_t kFrameBudget = 4ull << 30;
-uint32_t frame_capacity(const rtg_render_settings* cfg, uint32_t cap) {
-  const uint64_t per = frame_bytes_per_path(cfg);
-  if (!per) return cap;
-  while (cap > 1024 && (uint64_t)cap * per > kFrameBudget) cap >>= 1;
+uint32_t frame_capacity(const rtg_render_settings* cfg, uint32_t n_lights, uint32_t cap) {
+  while (cap > 1024 && chunk_extra_bytes(cfg, n_lights, cap) > kFrameBudget) cap >>= 1;
   return cap;
 }
 
@@ -1002,12 +1030,13 @@ rtg_status ensure_frames(rtg_context* ctx, uint64_t bytes) {
 }
 
 // The largest chunk (a power of two, at least one workgroup of paths) whose
-// path state and `per_path` more bytes per path fit `bytes`; 0 if none does.
-uint32_t static_capacity(bool f64, uint64_t bytes, uint64_t per_path) {
+// path state and `extra(cap)` more bytes fit `bytes`; 0 if none does.
+template <class Extra>
+uint32_t static_capacity(bool f64, uint64_t bytes, Extra extra) {
   for (uint32_t cap = 1u << 25; cap >= (uint32_t)kBlock; cap >>= 1) {
     const uint64_t paths = f64 ? lay_out_paths<double>(nullptr, cap, false, nullptr, nullptr)
                                : lay_out_paths<float>(nullptr, cap, false, nullptr, nullptr);
-    const uint64_t need = ((paths + 255) & ~uint64_t(255)) + per_path * cap;
+    const uint64_t need = ((paths + 255) & ~uint64_t(255)) + extra(cap);
     if (need <= bytes) return cap;
   }
   return 0;
@@ -1061,9 +1090,10 @@ rtg_status run_items(rtg_context* ctx, const PassItem* items, uint32_t n, const 
   // rays/s).
   auto cap_of = [&](uint32_t i) {
     const uint64_t paths = paths_of(i);
-    return frame_capacity(items[i].cfg, capacity_for(ctx, lanes_of(i) && paths >= (i + 1 < n ? chunk : split_last)
-                                                              ? (paths + 1) / 2
-                                                              : paths));
+    return frame_capacity(items[i].cfg, items[i].scene->dev.n_lights,
+                          capacity_for(ctx, lanes_of(i) && paths >= (i + 1 < n ? chunk : split_last)
+                                                ? (paths + 1) / 2
+                                                : paths));
   };
   uint32_t cap32 = 0, cap64 = 0;
   uint64_t frames = 0;
@@ -1071,7 +1101,7 @@ rtg_status run_items(rtg_context* ctx, const PassItem* items, uint32_t n, const 
   for (uint32_t i = 0; i < n; ++i) {
     uint32_t& c = items[i].cfg->precision == RTG_PRECISION_F64 ? cap64 : cap32;
     c = std::max(c, cap_of(i));
-    frames = std::max(frames, frame_bytes_per_path(items[i].cfg) * cap_of(i));
+    frames = std::max(frames, chunk_extra_bytes(items[i].cfg, items[i].scene->dev.n_lights, cap_of(i)));
     two = two || (lanes_of(i) && (n > 1 || paths_of(i) >= split_last));
   }
   if (frames)
@@ -1126,7 +1156,7 @@ rtg_status run_items(rtg_context* ctx, const PassItem* items, uint32_t n, const 
         P.counts = f64 ? B64.counts : B32.counts;
         P.stats = it.stats;
         P.frames = ctx->frame_mem;
-        P.frame_stride = cap;
+        set_direct(P, it.cfg, it.scene->dev.n_lights, cap);
         const hipStream_t st = lane2 ? ctx->stream2 : ctx->stream;
         const hipEvent_t wait = used[lane ^ 1] ? ctx->acc_done[lane ^ 1] : nullptr;
         const hipEvent_t rec = (two || cancel) ? ctx->acc_done[lane] : nullptr;
@@ -1301,7 +1331,7 @@ rtg_status run_group(rtg_context* ctx, const rtg_job* jobs, const Segment* seg, 
     P.shadow_counts = kBatchShadowCounts;
     P.stat_slots = kBatchStatSlots;
     P.zero_stats = 1;
-    P.counts = B.counts + (size_t)i * kBatchCounts;
+    P.counts = B.counts + (size_t)i * kBatchCounts * kCountStride;
     P.stats = B.stats + (size_t)i * kJobStatWords;
     pid += P.n_paths;
     q += (uint64_t)P.sub_cap * kSubQueues;
@@ -1423,9 +1453,9 @@ rtg_status rtg_render_pass_static(rtg_context* ctx, const rtg_scene* scene, cons
   const uint64_t slot_bytes = ((uint64_t)n_slots * 8 + 255) & ~uint64_t(255);
   unsigned long long* slots = reinterpret_cast<unsigned long long*>(ctx->scratch_dev);
   char* base = ctx->scratch_dev + slot_bytes;
-  const uint64_t frame_per_path = frame_bytes_per_path(cfg);
-  const uint32_t cap =
-      std::min(capacity_for(ctx, paths), static_capacity(f64, kStaticScratchBytes - slot_bytes, frame_per_path));
+  const uint32_t n_lights = scene->dev.n_lights;
+  auto extra = [&](uint32_t c) { return chunk_extra_bytes(cfg, n_lights, c); };
+  const uint32_t cap = std::min(capacity_for(ctx, paths), static_capacity(f64, kStaticScratchBytes - slot_bytes, extra));
   if (cap == 0)
     return fail(ctx, RTG_ERR_UNSUPPORTED,
                 fn() + ": the scratch array holds no chunk of 128 paths at this max_depth (" +
@@ -1434,7 +1464,8 @@ rtg_status rtg_render_pass_static(rtg_context* ctx, const rtg_scene* scene, cons
   PathBuffers<double> B64{};
   const uint64_t need = f64 ? lay_out_paths<double>(base, cap, false, slots, &B64)
                             : lay_out_paths<float>(base, cap, false, slots, &B32);
-  if (slot_bytes + ((need + 255) & ~uint64_t(255)) + frame_per_path * cap > kStaticScratchBytes)
+  const uint64_t extra_bytes = extra(cap);
+  if (slot_bytes + ((need + 255) & ~uint64_t(255)) + extra_bytes > kStaticScratchBytes)
     return fail(ctx, RTG_ERR_OUT_OF_MEMORY, fn() + ": chunk state exceeds the scratch array");
   char* frames = base + ((need + 255) & ~uint64_t(255));
   hipLaunchKernelGGL(k_zero_u64, dim3((n_slots + 127) / 128), dim3(128), 0, ctx->stream, slots, n_slots);
@@ -1465,8 +1496,8 @@ rtg_status rtg_render_pass_static(rtg_context* ctx, const rtg_scene* scene, cons
       P.stat_slots = kStatSlots;
       P.counts = f64 ? B64.counts : B32.counts;
       P.stats = slots;
-      P.frames = frame_per_path ? frames : nullptr;
-      P.frame_stride = cap;
+      P.frames = extra_bytes ? frames : nullptr;
+      set_direct(P, cfg, n_lights, cap);
       const hipEvent_t rec = cancel ? ctx->acc_done[0] : nullptr;
       if (f64)
         run_chunk(scene->dev, B64, P, scene->stack, ctx->stream, side_of(ctx), nullptr, nullptr, rec);

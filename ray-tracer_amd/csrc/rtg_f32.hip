@@ -9,11 +9,54 @@ namespace rtg {
 namespace {
 inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
+// Workgroups of `kernel` resident on the whole device at once.
+template <class K>
+uint32_t resident_blocks(K kernel) {
+  int dev = 0, cus = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, kBlock, 0) != hipSuccess || n < 1) n = 1;
+  return (uint32_t)n * (uint32_t)cus;
+}
+
 template <int STACK>
 void chunk_impl(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P, hipStream_t s,
                 const SideStream* side, const LaunchHooks* h, hipEvent_t acc_wait, hipEvent_t acc_done) {
   const uint32_t g = grid_of(P.n_paths);
-  if (P.integrator != RTG_INTEGRATOR_PATH) {
+  if (P.integrator == RTG_INTEGRATOR_DIRECT) {
+    // The weighted ray queue: generation 0 in one kernel, then one extend /
+    // shade / shadow round per depth (rounds past the last non-empty
+    // generation exit at once), the depth-first recomputation (which returns
+    // at once unless a generation overflowed) and the radiance.
+    DirectMem M;
+    direct_layout(static_cast<char*>(P.frames), P.frame_stride, P.dq_entries, S.n_lights, P.max_depth,
+                  P.dq_forced != 0, &M);
+    // Generations >= 1 run grid-stride over their queue with at most one
+    // device-full of workgroups: a generation's size is known only on the
+    // device, and a grid over every slot cost ~31 us per launch even when
+    // the generation was empty (8 Mi-sample chunks).
+    static const uint32_t r_ext = resident_blocks(k_dextend<STACK>), r_shade = resident_blocks(k_dshade<float>),
+                          r_shadow = resident_blocks(k_dshadow<STACK>), r_dfs = resident_blocks(k_ddfs<STACK>);
+    if (!M.forced) {
+      const uint32_t slots = std::max<uint32_t>(M.cap, (uint32_t)kSubQueues * M.sub_gen);
+      RTG_LAUNCH(h, RTG_K_GENERATE, k_dclear<float>, dim3(grid_of(dclear_threads())), dim3(kBlock), 0, s, M);
+      RTG_LAUNCH(h, RTG_K_EXTEND, (k_dfirst<STACK>), dim3(g), dim3(kBlock), 0, s, S, P, M);
+      for (int b = 1; b <= P.max_depth; ++b) {
+        RTG_LAUNCH(h, RTG_K_EXTEND, (k_dextend<STACK>), dim3(std::min(grid_of(slots), r_ext)), dim3(kBlock), 0, s, S,
+                   P, M, b);
+        RTG_LAUNCH(h, RTG_K_SHADE, k_dshade<float>, dim3(std::min(grid_of(slots), r_shade)), dim3(kBlock), 0, s, S, P,
+                   M, b);
+        if (S.n_lights && b < P.max_depth)
+          RTG_LAUNCH(h, RTG_K_SHADOW, (k_dshadow<STACK>),
+                     dim3(std::min(grid_of((uint64_t)kSubQueues * M.sub_lit), r_shadow)), dim3(kBlock), 0, s, S, P, M,
+                     b);
+      }
+    }
+    M.dfs_threads = std::min(M.dfs_threads, r_dfs * (uint32_t)kBlock);  // fewer threads, each with more samples
+    RTG_LAUNCH(h, RTG_K_DIRECT, (k_ddfs<STACK>), dim3(grid_of(M.dfs_threads)), dim3(kBlock), 0, s, S, P, M);
+    RTG_LAUNCH(h, RTG_K_ACCUMULATE, k_dfinish<float>, dim3(grid_of(dfinish_threads(P))), dim3(kBlock), 0, s, B, P, M);
+  } else if (P.integrator == RTG_INTEGRATOR_NORMALS) {
     RTG_LAUNCH(h, RTG_K_DIRECT, (k_direct32<STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P);
   } else {
     RTG_LAUNCH(h, RTG_K_GENERATE, k_generate<float>, dim3(grid_of(generate_threads(P))), dim3(kBlock), 0, s, S, B, P);

@@ -23,7 +23,10 @@ namespace rtg {
 
 constexpr int kBlock = 128;
 
-enum StatIdx { kStCamera = 0, kStRays, kStShadow, kStRR, kStInvalid, kStVisits, kStTests, kStSVisits, kStSTests, kStCount };
+// kStRedone: direct-integrator chunks recomputed depth first (their queue filled).
+enum StatIdx {
+  kStCamera = 0, kStRays, kStShadow, kStRR, kStInvalid, kStVisits, kStTests, kStSVisits, kStSTests, kStRedone, kStCount
+};
 
 template <class T>
 struct PathBuffers {
@@ -36,7 +39,7 @@ struct PathBuffers {
   int32_t* hplane;                 // hit plane (-1 none)
   uint64_t *rs, *ri;               // PCG32 state/inc (FP64 mode)
   uint32_t* q[2];                  // active queues (kSubQueues sub-queues of sub_cap each)
-  uint32_t* counts;                // sub-queue sizes: [(b)*kSubQueues + s] active, [(kMaxBounce+b)*kSubQueues + s] shadow
+  uint32_t* counts;                // sub-queue sizes (x kCountStride): [b*kSubQueues + s] active, [(kMaxBounce+b)*kSubQueues + s] shadow
   uint32_t* sq_pid;                // shadow queue (same sub-queue layout)
   T *sox, *soy, *soz, *sdx, *sdy, *sdz, *stm, *scx, *scy, *scz;
   // FP32: the shadow queue has two buffers (bounce parity; the second at
@@ -55,6 +58,15 @@ constexpr int kMaxBounce = 1024;
 // per sub-queue / slot, chosen by block index) so that no single address
 // takes an atomic from every block of a launch.
 constexpr int kSubQueues = 32;
+// Queue counters sit kCountStride words (64 bytes) apart. The sub-queue
+// counters take an atomic from every workgroup that appends; packed into one
+// cache line they serialise there (measured: the direct integrator's shade
+// kernel 19.9 -> 14.3 ms per sphere-field render with them apart; 16, 32,
+// 64 and 1024 words measure the same).
+#ifndef RTG_COUNT_STRIDE
+#define RTG_COUNT_STRIDE 16
+#endif
+constexpr uint32_t kCountStride = RTG_COUNT_STRIDE;
 constexpr int kStatSlots = 256;
 constexpr int kStatStride = 16;  // counters per slot (128-byte lines)
 __host__ __device__ constexpr uint32_t sub_queue_cap(uint32_t capacity) {
@@ -80,10 +92,12 @@ struct ChunkParams {
   uint32_t shadow_counts;  // offset of the shadow-queue counters in `counts`
   uint32_t stat_slots;     // statistics slots of the job
   uint32_t zero_stats;     // generate clears the job's statistics slots
-  uint32_t* counts;        // queue sizes: [b*kSubQueues + s] active, [shadow_counts + b*kSubQueues + s] shadow
+  uint32_t* counts;        // queue sizes (x kCountStride): [b*kSubQueues + s] active, [shadow_counts + b*kSubQueues + s] shadow
   unsigned long long* stats;
-  void* frames;            // direct integrator: the recursion's frames (kFrameWords x max_depth per path)
+  void* frames;            // direct integrator: FP64 recursion frames (kFrameWords x max_depth per path), FP32 queues
   uint32_t frame_stride;   // paths per frame field (the chunk capacity)
+  uint32_t dq_entries;     // FP32 direct: ray slots of one generation
+  uint32_t dq_forced;      // FP32 direct: depth first from the start
 };
 
 // ---------------------------------------------------------------------------
@@ -118,7 +132,7 @@ __device__ __forceinline__ AppendSlots block_append2(uint32_t* counts_a, uint32_
     uint32_t tot = 0;
     for (uint32_t w = 0; w < kBlock / 64; ++w)
       tot += threadIdx.x == 0 ? s_wave[0][w] : s_wave[1][w] + s_wave[2][w];
-    uint32_t* c = (threadIdx.x == 0 ? counts_a : counts_b) + sub;
+    uint32_t* c = (threadIdx.x == 0 ? counts_a : counts_b) + (size_t)sub * kCountStride;
     s_base[threadIdx.x] = tot ? atomicAdd(c, tot) : 0u;
   }
   __syncthreads();
@@ -148,10 +162,13 @@ struct QueueView {
 // and a shuffle scan), so both waves of a block agree on the total without
 // LDS or a barrier, and blocks past the end of the queue (the grid covers
 // every path, the queue shrinks with each bounce) exit at once. Other blocks
-// publish the exclusive prefix sums in pre[] for queue_slot.
-__device__ __forceinline__ QueueView load_queue(const uint32_t* counts, uint32_t* pre, uint32_t lb) {
+// publish the exclusive prefix sums in pre[] for queue_slot. A sub-queue
+// holds at most `cap` entries (appends past it are dropped by the producer).
+__device__ __forceinline__ QueueView load_queue(const uint32_t* counts, uint32_t* pre, uint32_t lb,
+                                                uint32_t cap = ~0u, uint32_t stride = 1) {
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t c = lane < (uint32_t)kSubQueues ? counts[lane] : 0u;
+  uint32_t c = lane < (uint32_t)kSubQueues ? counts[lane * stride] : 0u;
+  c = c < cap ? c : cap;
   uint32_t x = c;
   for (int off = 1; off < kSubQueues; off <<= 1) {
     const uint32_t y = __shfl_up(x, off, 64);
@@ -307,8 +324,8 @@ __device__ __forceinline__ void generate_body(const DevScene& S, const PathBuffe
   const uint32_t l = lb * kBlock + threadIdx.x;
   const uint32_t nz = (uint32_t)(P.max_depth + 1) * kSubQueues;
   if (l < nz) {
-    P.counts[l] = l == 0 ? P.n_paths : 0u;
-    P.counts[P.shadow_counts + l] = 0u;
+    P.counts[(size_t)l * kCountStride] = l == 0 ? P.n_paths : 0u;
+    P.counts[(size_t)(P.shadow_counts + l) * kCountStride] = 0u;
   }
   if (P.zero_stats && l < P.stat_slots * kStatStride) P.stats[l] = 0;
   if (l >= P.n_paths) return;
@@ -356,7 +373,7 @@ __device__ __forceinline__ uint32_t active_pid(const PathBuffers<T>& B, const Ch
 template <class T, int STACK>
 __device__ __forceinline__ void extend_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P, int b,
                                             uint32_t lb, uint32_t* sref, uint32_t* pre) {
-  const QueueView qv = load_queue(P.counts + (size_t)b * kSubQueues, pre, lb);
+  const QueueView qv = load_queue(P.counts + (size_t)b * kSubQueues * kCountStride, pre, lb, ~0u, kCountStride);
   if (qv.past_end) return;
   const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
@@ -387,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene S, PathBuffers<T> B,
 template <class T>
 __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P, int b,
                                            uint32_t lb, uint32_t* pre) {
-  const QueueView qv = load_queue(P.counts + (size_t)b * kSubQueues, pre, lb);
+  const QueueView qv = load_queue(P.counts + (size_t)b * kSubQueues * kCountStride, pre, lb, ~0u, kCountStride);
   if (qv.past_end) return;
   const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
@@ -450,8 +467,9 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
     B.lx[pid] = L.x; B.ly[pid] = L.y; B.lz[pid] = L.z;
   }
   // Converged: compacted appends to the next active queue and the shadow queue.
-  const AppendSlots a = block_append2(P.counts + (size_t)(b + 1) * kSubQueues,
-                                      P.counts + P.shadow_counts + (size_t)b * kSubQueues, P.sub_cap, alive, shadow,
+  const AppendSlots a = block_append2(P.counts + (size_t)(b + 1) * kSubQueues * kCountStride,
+                                      P.counts + (size_t)(P.shadow_counts + b * kSubQueues) * kCountStride, P.sub_cap,
+                                      alive, shadow,
                                       shadow_cls, lb);
   if (alive) B.q[(b + 1) & 1][P.q_base + a.a] = pid;
   if (shadow) {
@@ -475,7 +493,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, PathBuffers<T> B, 
 template <class T, int STACK>
 __device__ __forceinline__ void shadow_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P, int b,
                                             uint32_t lb, uint32_t* sref, uint32_t* pre) {
-  const QueueView qv = load_queue(P.counts + P.shadow_counts + (size_t)b * kSubQueues, pre, lb);
+  const QueueView qv =
+      load_queue(P.counts + (size_t)(P.shadow_counts + b * kSubQueues) * kCountStride, pre, lb, ~0u, kCountStride);
   if (qv.past_end) return;
   const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
@@ -909,9 +928,9 @@ __global__ __launch_bounds__(kMaxBatchJobs) void k_batch_plan(const BatchJob* __
   if (j < n_jobs) {
     const ChunkParams& P = jobs[j].P;
     if (b + 1 < P.max_depth)
-      for (int s = 0; s < kSubQueues; ++s) na += P.counts[(size_t)(b + 1) * kSubQueues + s];
+      for (int s = 0; s < kSubQueues; ++s) na += P.counts[((size_t)(b + 1) * kSubQueues + s) * kCountStride];
     if (b >= 0 && b < P.max_depth)
-      for (int s = 0; s < kSubQueues; ++s) ns += P.counts[P.shadow_counts + (size_t)b * kSubQueues + s];
+      for (int s = 0; s < kSubQueues; ++s) ns += P.counts[((size_t)P.shadow_counts + b * kSubQueues + s) * kCountStride];
   }
   sa[j] = blocks_of(na);
   ss[j] = blocks_of(ns);

@@ -4,6 +4,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include "rtg_direct.hpp"
 #include "rtg_kernels.hpp"
 
 namespace rtg {
@@ -53,7 +54,8 @@ struct SideStream {
 };
 
 // Runs one chunk of a render pass: generate, max_depth x (extend, shade,
-// shadow), accumulate; or the per-sample direct/normals kernel. The
+// shadow), accumulate; FP32 direct: the weighted ray queue (rtg_direct.hpp);
+// FP64 direct and normals: the per-sample kernel. The
 // accumulation waits for `acc_wait` (if not null: another lane's latest
 // accumulation) and records `acc_done` (if not null).
 void run_chunk(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P, uint32_t stack, hipStream_t s,

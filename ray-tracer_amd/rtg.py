@@ -77,7 +77,7 @@ class RenderSettings(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("camera_samples", C.c_uint64), ("rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("rr_terminated", C.c_uint64), ("invalid_samples", C.c_uint64), ("node_visits", C.c_uint64),
-                ("prim_tests", C.c_uint64), ("elapsed_ms", C.c_double)]
+                ("prim_tests", C.c_uint64), ("elapsed_ms", C.c_double), ("redone_chunks", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
