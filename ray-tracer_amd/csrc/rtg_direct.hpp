@@ -8,8 +8,9 @@
 // Expanded, a sample's radiance is the sum over the tree's nodes of (the
 // product of the weights from the camera ray down to the node) x (the node's
 // own terms). It is evaluated here breadth first, like the path integrator's
-// wavefront: generation g holds the rays at depth g, each with its weight,
-// and per generation
+// wavefront: generation g holds the rays at depth g, each with its weight.
+// Generation 0 — the camera rays, coherent, one per sample — runs as one
+// kernel that also traces its shadow rays (k_dfirst); per later generation
 //   extend  closest hit of every ray                                (k_dextend)
 //   shade   the node's emission / sky term; at a diffuse hit one shadow
 //           request per light (the hit joins the lit queue); the children
@@ -351,8 +352,8 @@ __host__ __device__ inline uint32_t dclear_threads() {
 // node is its sample's only one: its sums are stored. Camera rays of
 // neighbouring pixels, and their shadow rays to one light, are coherent, so
 // one thread per sample fits here; the queue takes over where trees branch
-// (on the torus workload the separate generation-0 kernels took 27.9 ms per
-// render, this one with the queue after it 19.x).
+// (torus workload, direct: 27.9 ms per render with generation 0 as separate
+// extend / shade / shadow kernels, 22.8 ms with this kernel).
 template <int STACK>
 __global__ __launch_bounds__(kBlock) void k_dfirst(DevScene S, ChunkParams P, DirectMem M) {
   __shared__ uint32_t sref[STACK * kBlock];

@@ -14,8 +14,10 @@
 //               wavefront, ballot + mbcnt for the lane offsets)
 //     shadow    any-hit of the shadow queue; unoccluded -> L += contribution
 //   accumulate per-pixel sums in sample order (framebuffer.hpp:31-35)
-// The direct and normals integrators run as one kernel per sample (their
-// recursion is a depth-first walk with an explicit frame stack).
+// FP32 direct runs as a weighted ray queue (rtg_direct.hpp). FP64 direct
+// (which must draw the reference's PCG32 stream in its depth-first order) and
+// normals run one kernel per sample (direct's recursion a depth-first walk
+// over an explicit frame stack).
 #pragma once
 #include "rtg_shade.hpp"
 
@@ -619,7 +621,8 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(PathBuffers<T> B, ChunkPa
 }
 
 // ---------------------------------------------------------------------------
-// direct / normals integrators: one sample per thread (integrator.cpp:141-198).
+// direct / normals integrators: one sample per thread (integrator.cpp:141-198);
+// FP64 direct and normals in either precision (FP32 direct: rtg_direct.hpp).
 //
 // direct_trace recurses: a metal hit adds albedo x (its reflection's
 // radiance), a dielectric hit tint x F x (reflection) + tint x (1 - F) x
