@@ -69,8 +69,12 @@ void chunk_impl(const DevScene& S, const PathBuffers<float>& B, const ChunkParam
     // waits for the any-hit kernel of bounce b - 2; the accumulation waits
     // for the last. Every path's additions keep their order.
     bool pending[2] = {false, false};
+    // The traversal kernels are persistent (at most one device-full of
+    // workgroups, idle lanes refilled between rounds of their walks).
+    static const uint32_t r_ext = resident_blocks(k_extend_dyn<STACK>), r_sh = resident_blocks(k_shadow_dyn<STACK>);
+    const uint32_t g_ext = std::min(g, r_ext), g_sh = std::min(g, r_sh);
     for (int b = 0; b < P.max_depth; ++b) {
-      RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+      RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend_dyn<STACK>), dim3(g_ext), dim3(kBlock), 0, s, S, B, P, b);
       if (pending[b & 1]) hipStreamWaitEvent(s, side->shadowed[b & 1], 0);
       pending[b & 1] = false;
       RTG_LAUNCH(h, RTG_K_SHADE, k_shade<float>, dim3(g), dim3(kBlock), 0, s, S, B, P, b);
@@ -78,11 +82,11 @@ void chunk_impl(const DevScene& S, const PathBuffers<float>& B, const ChunkParam
       if (side) {
         hipEventRecord(side->shaded, s);
         hipStreamWaitEvent(side->stream, side->shaded, 0);
-        RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<float, STACK>), dim3(g), dim3(kBlock), 0, side->stream, S, B, P, b);
+        RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow_dyn<STACK>), dim3(g_sh), dim3(kBlock), 0, side->stream, S, B, P, b);
         hipEventRecord(side->shadowed[b & 1], side->stream);
         pending[b & 1] = true;
       } else {
-        RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
+        RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow_dyn<STACK>), dim3(g_sh), dim3(kBlock), 0, s, S, B, P, b);
       }
     }
     for (int p = 0; p < 2; ++p)
@@ -104,7 +108,7 @@ void batch_impl(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPl
   // bounce b - 2.
   bool pending[2] = {false, false};
   for (int b = 0; b < depth; ++b) {
-    RTG_LAUNCH(h, RTG_K_EXTEND, (k_batch_extend<float, STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
+    RTG_LAUNCH(h, RTG_K_EXTEND, (k_wide_extend_dyn<STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
     if (pending[b & 1]) hipStreamWaitEvent(s, side->shadowed[b & 1], 0);
     pending[b & 1] = false;
     RTG_LAUNCH(h, RTG_K_SHADE, k_batch_shade<float>, dim3(g.shade), dim3(kBlock), 0, s, jobs, B, L, b);
@@ -112,12 +116,12 @@ void batch_impl(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPl
     if (side) {
       hipEventRecord(side->shaded, s);
       hipStreamWaitEvent(side->stream, side->shaded, 0);
-      RTG_LAUNCH(h, RTG_K_SHADOW, (k_batch_shadow<float, STACK>), dim3(g.trace), dim3(kBlock), 0, side->stream, jobs,
-                 B, L, b);
+      RTG_LAUNCH(h, RTG_K_SHADOW, (k_wide_shadow_dyn<STACK>), dim3(g.trace), dim3(kBlock), 0, side->stream, jobs, B,
+                 L, b);
       hipEventRecord(side->shadowed[b & 1], side->stream);
       pending[b & 1] = true;
     } else {
-      RTG_LAUNCH(h, RTG_K_SHADOW, (k_batch_shadow<float, STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
+      RTG_LAUNCH(h, RTG_K_SHADOW, (k_wide_shadow_dyn<STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
     }
   }
   for (int p = 0; p < 2; ++p)
@@ -132,9 +136,13 @@ void wide_impl(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPla
                hipStream_t s, const SideStream* side, const LaunchHooks* h, hipEvent_t acc_wait, hipEvent_t acc_done) {
   RTG_LAUNCH(h, RTG_K_GENERATE, k_wide_generate<float>, dim3(g.generate), dim3(kBlock), 0, s, jobs, B, L);
   RTG_LAUNCH(h, RTG_K_PLAN, k_batch_plan<float>, dim3(1), dim3(kMaxBatchJobs), 0, s, jobs, L, -1);
+  // Traversal as in a single render: persistent workgroups, idle lanes refilled.
+  static const uint32_t r_ext = resident_blocks(k_wide_extend_dyn<STACK>),
+                        r_sh = resident_blocks(k_wide_shadow_dyn<STACK>);
+  const uint32_t g_ext = std::min(g.trace, r_ext), g_sh = std::min(g.trace, r_sh);
   bool pending[2] = {false, false};
   for (int b = 0; b < depth; ++b) {
-    RTG_LAUNCH(h, RTG_K_EXTEND, (k_wide_extend<float, STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
+    RTG_LAUNCH(h, RTG_K_EXTEND, (k_wide_extend_dyn<STACK>), dim3(g_ext), dim3(kBlock), 0, s, jobs, B, L, b);
     if (pending[b & 1]) hipStreamWaitEvent(s, side->shadowed[b & 1], 0);
     pending[b & 1] = false;
     RTG_LAUNCH(h, RTG_K_SHADE, k_wide_shade<float>, dim3(g.shade), dim3(kBlock), 0, s, jobs, B, L, b);
@@ -142,12 +150,12 @@ void wide_impl(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPla
     if (side) {
       hipEventRecord(side->shaded, s);
       hipStreamWaitEvent(side->stream, side->shaded, 0);
-      RTG_LAUNCH(h, RTG_K_SHADOW, (k_wide_shadow<float, STACK>), dim3(g.trace), dim3(kBlock), 0, side->stream, jobs,
-                 B, L, b);
+      RTG_LAUNCH(h, RTG_K_SHADOW, (k_wide_shadow_dyn<STACK>), dim3(g_sh), dim3(kBlock), 0, side->stream, jobs, B, L,
+                 b);
       hipEventRecord(side->shadowed[b & 1], side->stream);
       pending[b & 1] = true;
     } else {
-      RTG_LAUNCH(h, RTG_K_SHADOW, (k_wide_shadow<float, STACK>), dim3(g.trace), dim3(kBlock), 0, s, jobs, B, L, b);
+      RTG_LAUNCH(h, RTG_K_SHADOW, (k_wide_shadow_dyn<STACK>), dim3(g_sh), dim3(kBlock), 0, s, jobs, B, L, b);
     }
   }
   for (int p = 0; p < 2; ++p)
@@ -160,9 +168,9 @@ void wide_impl(const BatchJob* jobs, const PathBuffers<float>& B, const BatchPla
 template <int STACK>
 uint32_t occupancy_trace() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_extend<float, STACK>, kBlock, 0) != hipSuccess) n = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_wide_extend_dyn<STACK>, kBlock, 0) != hipSuccess) n = 1;
   int m = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k_batch_shadow<float, STACK>, kBlock, 0) != hipSuccess) m = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k_wide_shadow_dyn<STACK>, kBlock, 0) != hipSuccess) m = 1;
   return (uint32_t)std::max(1, std::min(n, m));
 }
 

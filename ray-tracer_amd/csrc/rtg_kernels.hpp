@@ -529,6 +529,162 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, PathBuffers<T> B,
   shadow_body<T, STACK>(S, B, P, b, blockIdx.x, sref, pre);
 }
 
+// Persistent FP32 walks of a bounce's closest-hit and any-hit queues. The
+// grid holds at most what the device runs at once; each workgroup takes a
+// contiguous range of the queue (whole workgroups of rays), and between
+// rounds of its walk (rtg_trace.hpp) a wave hands its idle lanes the range's
+// next rays — after Aila & Laine 2009, replacing terminated rays — so lanes
+// whose rays end early keep working while the longest walks of their wave
+// finish. A ray's walk and counters depend on the ray alone: results are
+// bitwise those of one ray per thread (k_extend / k_shadow).
+constexpr uint32_t kRefill = 32;  // a wave takes new rays once half its lanes are idle
+// A workgroup's contiguous range of the queue (whole workgroups of rays).
+// (Interleaving the workgroups' 128-ray chunks instead, so that they all run
+// out together, measured the same.)
+struct Range {
+  uint32_t r0, r1;
+};
+__device__ __forceinline__ Range block_range(uint32_t total) {
+  const uint32_t per = (uint32_t)(((uint64_t)total + (uint64_t)gridDim.x * kBlock - 1) / ((uint64_t)gridDim.x * kBlock)) * kBlock;
+  const uint64_t r0 = (uint64_t)blockIdx.x * per;
+  const uint32_t a = r0 < total ? (uint32_t)r0 : total;
+  return Range{a, total - a < per ? total : a + per};
+}
+// Next ray indices for the idle lanes of a converged wave: true while the
+// workgroup's range lasts; `i` the lane's queue index (>= r1 when none).
+__device__ __forceinline__ bool refill(uint32_t* cursor, const Range& rg, unsigned long long idle, bool& more,
+                                       uint32_t& i) {
+  const uint32_t n_idle = (uint32_t)__popcll(idle);
+  if (!more || (n_idle < kRefill && n_idle != 64u)) return false;
+  uint32_t base = 0;
+  if ((threadIdx.x & 63u) == 0) base = atomicAdd(cursor, n_idle);
+  base = __shfl(base, 0, 64);
+  if (base >= rg.r1) {
+    more = false;
+    return false;
+  }
+  i = base + lanes_below(idle);
+  return true;
+}
+
+// Closest hits of queue indices [rg.r0, rg.r1) of bounce b (`cursor`, in
+// LDS, starts at rg.r0), idle lanes refilled between rounds; adds rays,
+// node visits and primitive tests to st.
+template <int STACK>
+__device__ __forceinline__ void extend_range(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P,
+                                             int b, uint32_t* sref, const uint32_t* pre, uint32_t* cursor,
+                                             const Range& rg, uint32_t* st) {
+  const LdsStack stk = lds_stack(sref);
+  Counters c{0, 0};
+  Trav32 T;
+  T.w = Walk{kNoRef, kNoRef, 0, 0};
+  Ray32 r{};
+  uint32_t pid = 0;
+  bool active = false, more = true;
+  auto finish = [&]() {
+    B.ht[pid] = T.h.t;
+    B.hb1[pid] = T.h.b1;
+    B.hb2[pid] = T.h.b2;
+    B.hprim[pid] = T.h.prim;
+    B.hplane[pid] = T.h.plane;
+  };
+  for (;;) {
+    uint32_t i = 0;
+    if (refill(cursor, rg, __ballot(!active), more, i) && !active && i < rg.r1) {
+      pid = active_pid(B, P, pre, b, i);
+      r = make_ray32(B.ox[pid], B.oy[pid], B.oz[pid], B.dx[pid], B.dy[pid], B.dz[pid]);
+      ++st[kStRays];
+      active = closest_begin32(S, r, 1e-6f, __builtin_inff(), stk, c, T);
+      if (!active) finish();
+    }
+    if (!__ballot(active)) {
+      if (!more) break;
+      continue;
+    }
+    if (active && closest_round32(S, r, 1e-6f, stk, c, T)) {
+      finish();
+      active = false;
+    }
+  }
+  st[kStVisits] += c.visits;
+  st[kStTests] += c.tests;
+}
+
+// Any hits of shadow-queue indices [rg.r0, rg.r1) of bounce b, as above;
+// an unoccluded request adds its contribution to its path's shadow radiance.
+template <int STACK>
+__device__ __forceinline__ void shadow_range(const DevScene& S, const PathBuffers<float>& B, const ChunkParams& P,
+                                             int b, uint32_t* sref, const uint32_t* pre, uint32_t* cursor,
+                                             const Range& rg, uint32_t* st) {
+  const LdsStack stk = lds_stack(sref);
+  Counters c{0, 0};
+  Walk w{kNoRef, kNoRef, 0, 0};
+  Ray32 r{};
+  uint32_t k = 0, neg = 0;
+  float tmax = 0.0f;
+  bool active = false, more = true, hit = false;
+  auto finish = [&]() {
+    if (!hit) {
+      const uint32_t pid = B.sq_pid[k];
+      B.slx[pid] = B.slx[pid] + B.scx[k];
+      B.sly[pid] = B.sly[pid] + B.scy[k];
+      B.slz[pid] = B.slz[pid] + B.scz[k];
+    }
+  };
+  for (;;) {
+    uint32_t i = 0;
+    if (refill(cursor, rg, __ballot(!active), more, i) && !active && i < rg.r1) {
+      k = P.q_base + queue_slot(pre, P.sub_cap, i) + (uint32_t)(b & 1) * B.sq_stride;
+      r = make_ray32(B.sox[k], B.soy[k], B.soz[k], B.sdx[k], B.sdy[k], B.sdz[k]);
+      tmax = B.stm[k];
+      active = any_begin32(S, r, 1e-6f, tmax, stk, c, w, neg, hit);
+      if (!active) finish();
+    }
+    if (!__ballot(active)) {
+      if (!more) break;
+      continue;
+    }
+    if (active && any_round32(S, r, 1e-6f, tmax, stk, c, w, neg, hit)) {
+      finish();
+      active = false;
+    }
+  }
+  st[kStSVisits] += c.visits;
+  st[kStSTests] += c.tests;
+}
+
+// 72 VGPRs (7 waves per SIMD); held to 64 it spills and runs slower.
+template <int STACK>
+__global__ __launch_bounds__(kBlock) void k_extend_dyn(DevScene S, PathBuffers<float> B, ChunkParams P, int b) {
+  __shared__ uint32_t sref[STACK * kBlock];
+  __shared__ uint32_t pre[kSubQueues + 1];
+  __shared__ uint32_t cursor;
+  const QueueView qv = load_queue(P.counts + (size_t)b * kSubQueues * kCountStride, pre, blockIdx.x, ~0u, kCountStride);
+  if (qv.past_end) return;
+  const Range rg = block_range(qv.total);
+  if (threadIdx.x == 0) cursor = rg.r0;
+  __syncthreads();
+  uint32_t st[kStCount] = {0};
+  extend_range<STACK>(S, B, P, b, sref, pre, &cursor, rg, st);
+  flush_stats(P, st, blockIdx.x);
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kBlock) void k_shadow_dyn(DevScene S, PathBuffers<float> B, ChunkParams P, int b) {
+  __shared__ uint32_t sref[STACK * kBlock];
+  __shared__ uint32_t pre[kSubQueues + 1];
+  __shared__ uint32_t cursor;
+  const QueueView qv = load_queue(P.counts + (size_t)(P.shadow_counts + b * kSubQueues) * kCountStride, pre,
+                                  blockIdx.x, ~0u, kCountStride);
+  if (qv.past_end) return;
+  const Range rg = block_range(qv.total);
+  if (threadIdx.x == 0) cursor = rg.r0;
+  __syncthreads();
+  uint32_t st[kStCount] = {0};
+  shadow_range<STACK>(S, B, P, b, sref, pre, &cursor, rg, st);
+  flush_stats(P, st, blockIdx.x);
+}
+
 // accumulate: the invalid-sample guard and Framebuffer::add in sample order
 // (renderer.cpp:39-45, framebuffer.hpp:31-35).
 //
@@ -869,9 +1025,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const double* __re
 // Batches: many jobs (different scenes, resolutions, sample ranges, depths)
 // share one set of path arrays and one launch per step. A plan holds, per
 // job, the first virtual block of the job (prefix sums, n_jobs + 1 entries).
-// Workgroups are persistent and walk the virtual blocks round-robin, so each
-// virtual block belongs to one job: the job's scene and settings are uniform
-// across the workgroup and sit in scalar registers.
+// Workgroups are persistent and walk the virtual blocks round-robin (the
+// traversal kernels walk contiguous shares job by job: k_wide_*_dyn), so
+// each virtual block belongs to one job: the job's scene and settings are
+// uniform across the workgroup and sit in scalar registers.
 struct BatchJob {
   DevScene S;
   ChunkParams P;
@@ -962,31 +1119,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_generate(const BatchJob* __res
   batch_loop(L.gen, jobs_of(L), [&](uint32_t j, uint32_t lb) { generate_body<T>(jobs[j].S, B, jobs[j].P, lb); });
 }
 
-template <class T, int STACK>
-__global__ __launch_bounds__(kBlock) void k_batch_extend(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
-                                                         BatchPlans L, int b) {
-  __shared__ uint32_t sref[STACK * kBlock];
-  __shared__ uint32_t pre[kSubQueues + 1];
-  batch_loop(L.act, jobs_of(L), [&](uint32_t j, uint32_t lb) {
-    extend_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, pre);
-  });
-}
-
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_batch_shade(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
                                                         BatchPlans L, int b) {
   __shared__ uint32_t pre[kSubQueues + 1];
   batch_loop(L.act, jobs_of(L), [&](uint32_t j, uint32_t lb) { shade_body<T>(jobs[j].S, B, jobs[j].P, b, lb, pre); });
-}
-
-template <class T, int STACK>
-__global__ __launch_bounds__(kBlock) void k_batch_shadow(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
-                                                         BatchPlans L, int b) {
-  __shared__ uint32_t sref[STACK * kBlock];
-  __shared__ uint32_t pre[kSubQueues + 1];
-  batch_loop(L.shd + (b & 1) * (kMaxBatchJobs + 1), jobs_of(L), [&](uint32_t j, uint32_t lb) {
-    shadow_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, pre);
-  });
 }
 
 template <class T>
@@ -1016,16 +1153,6 @@ __global__ __launch_bounds__(kBlock) void k_wide_generate(const BatchJob* __rest
   batch_block(L.gen, jobs_of(L), [&](uint32_t j, uint32_t lb) { generate_body<T>(jobs[j].S, B, jobs[j].P, lb); });
 }
 
-template <class T, int STACK>
-__global__ __launch_bounds__(kBlock) void k_wide_extend(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
-                                                        BatchPlans L, int b) {
-  __shared__ uint32_t sref[STACK * kBlock];
-  __shared__ uint32_t pre[kSubQueues + 1];
-  batch_block(L.act, jobs_of(L), [&](uint32_t j, uint32_t lb) {
-    extend_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, pre);
-  });
-}
-
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_wide_shade(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
                                                        BatchPlans L, int b) {
@@ -1033,13 +1160,64 @@ __global__ __launch_bounds__(kBlock) void k_wide_shade(const BatchJob* __restric
   batch_block(L.act, jobs_of(L), [&](uint32_t j, uint32_t lb) { shade_body<T>(jobs[j].S, B, jobs[j].P, b, lb, pre); });
 }
 
-template <class T, int STACK>
-__global__ __launch_bounds__(kBlock) void k_wide_shadow(const BatchJob* __restrict__ jobs, PathBuffers<T> B,
-                                                        BatchPlans L, int b) {
+// Wide batch passes with the persistent traversal of a single render: a
+// workgroup takes a contiguous share of the plan's virtual blocks and walks
+// it job by job (a job's scene and settings stay uniform across the
+// workgroup); within a job its idle lanes are refilled between rounds, and
+// at the job's end the workgroup drains before taking the next job.
+template <class F>
+__device__ __forceinline__ void wide_segments(const uint32_t* __restrict__ plan, uint32_t n_jobs, F&& f) {
+  const uint32_t total = __builtin_amdgcn_readfirstlane(plan[n_jobs]);
+  const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
+  uint32_t vb = blockIdx.x * per;
+  const uint32_t vend = vb + per < total ? vb + per : total;
+  while (vb < vend) {
+    const uint32_t j = job_of(plan, n_jobs, vb);
+    const uint32_t jb0 = __builtin_amdgcn_readfirstlane(plan[j]), jb1 = __builtin_amdgcn_readfirstlane(plan[j + 1]);
+    const uint32_t seg = jb1 < vend ? jb1 : vend;
+    f(j, vb - jb0, seg - jb0);  // the job's virtual blocks [a, b)
+    vb = seg;
+  }
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kBlock) void k_wide_extend_dyn(const BatchJob* __restrict__ jobs, PathBuffers<float> B,
+                                                            BatchPlans L, int b) {
   __shared__ uint32_t sref[STACK * kBlock];
   __shared__ uint32_t pre[kSubQueues + 1];
-  batch_block(L.shd + (b & 1) * (kMaxBatchJobs + 1), jobs_of(L), [&](uint32_t j, uint32_t lb) {
-    shadow_body<T, STACK>(jobs[j].S, B, jobs[j].P, b, lb, sref, pre);
+  __shared__ uint32_t cursor;
+  wide_segments(L.act, jobs_of(L), [&](uint32_t j, uint32_t a, uint32_t e) {
+    const DevScene& S = jobs[j].S;
+    const ChunkParams& P = jobs[j].P;
+    const QueueView qv = load_queue(P.counts + (size_t)b * kSubQueues * kCountStride, pre, a, ~0u, kCountStride);
+    if (qv.past_end) return;
+    const Range rg{a * kBlock, e * kBlock < qv.total ? e * kBlock : qv.total};
+    if (threadIdx.x == 0) cursor = rg.r0;
+    __syncthreads();
+    uint32_t st[kStCount] = {0};
+    extend_range<STACK>(S, B, P, b, sref, pre, &cursor, rg, st);
+    flush_stats(P, st, a);  // ends with a barrier: pre and cursor are free for the next job
+  });
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kBlock) void k_wide_shadow_dyn(const BatchJob* __restrict__ jobs, PathBuffers<float> B,
+                                                            BatchPlans L, int b) {
+  __shared__ uint32_t sref[STACK * kBlock];
+  __shared__ uint32_t pre[kSubQueues + 1];
+  __shared__ uint32_t cursor;
+  wide_segments(L.shd + (b & 1) * (kMaxBatchJobs + 1), jobs_of(L), [&](uint32_t j, uint32_t a, uint32_t e) {
+    const DevScene& S = jobs[j].S;
+    const ChunkParams& P = jobs[j].P;
+    const QueueView qv = load_queue(P.counts + (size_t)(P.shadow_counts + b * kSubQueues) * kCountStride, pre, a,
+                                    ~0u, kCountStride);
+    if (qv.past_end) return;
+    const Range rg{a * kBlock, e * kBlock < qv.total ? e * kBlock : qv.total};
+    if (threadIdx.x == 0) cursor = rg.r0;
+    __syncthreads();
+    uint32_t st[kStCount] = {0};
+    shadow_range<STACK>(S, B, P, b, sref, pre, &cursor, rg, st);
+    flush_stats(P, st, a);
   });
 }
 

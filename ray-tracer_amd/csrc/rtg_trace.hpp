@@ -520,14 +520,84 @@ struct Walk {
   __device__ __forceinline__ bool descended(uint32_t steps) const { return leaf != kNoRef && steps >= kDescent; }
 };
 
+// A ray's walk in rounds (one descent, then the leaves it parked), so that a
+// kernel may hand a lane whose walk has ended a new ray between rounds:
+// begin once, then round until it returns true. A ray's rounds depend on the
+// ray alone, whatever its lane does before or after it.
+struct Trav32 {
+  Hit32 h;
+  Walk w;
+  uint32_t neg;  // direction signs (bit per axis)
+};
+// Closest hit: planes, then the root box; false when that already ends the
+// walk (h final).
+__device__ __forceinline__ bool closest_begin32(const DevScene& S, const Ray32& r, float tmin, float tmax,
+                                                LdsStack stk, Counters& c, Trav32& T) {
+  T.h.plane = planes32(S, r, tmin, tmax);
+  T.h.t = tmax;
+  T.h.prim = 0xFFFFFFFFu;
+  T.h.b1 = T.h.b2 = 0.0f;
+  T.w = Walk{kNoRef, kNoRef, 0, 0};
+  T.neg = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
+  if (!S.has_bvh) return false;
+  float tn;
+  ++c.visits;
+  if (!box32(r, S.root_lo[0], S.root_lo[1], S.root_lo[2], S.root_hi[0], S.root_hi[1], S.root_hi[2], tmin, T.h.t, tn))
+    return false;
+  T.w.ref = S.root_ref;
+  T.w.park(stk);
+  return true;
+}
+// One round of the closest-hit walk; true when the walk has ended. A lane
+// with no walk (ref and leaf kNoRef) passes through.
+__device__ __forceinline__ bool closest_round32(const DevScene& S, const Ray32& r, float tmin, LdsStack stk,
+                                                Counters& c, Trav32& T) {
+  Walk& w = T.w;
+  Hit32& h = T.h;
+  for (uint32_t steps = 1; w.inner(); ++steps) {
+    const uint32_t under = w.below(stk);
+    const float4* n = S.node2 + (size_t)w.ref * kNode2Stride;
+    const float4 a = ld_global(n), b = ld_global(n + 1), cc = ld_global(n + 2);
+    const uint4 m = ld_global(reinterpret_cast<const uint4*>(n + 3));
+    float tl, tr;
+    const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, h.t, tl);
+    const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, h.t, tr);
+    c.visits += 2;
+    const bool right_first = (T.neg >> m.z) & 1u;
+    const uint32_t first = right_first ? m.y : m.x, second = right_first ? m.x : m.y;
+    const bool hf = right_first ? hr : hl, hs = right_first ? hl : hr;
+    w.step(stk, under, first, second, hf, hs);
+    if (w.descended(steps)) break;
+  }
+  w.park(stk);  // a leaf popped after a miss
+  while (w.leaf != kNoRef) {
+    const uint32_t k0 = leaf_first(w.leaf), kn = k0 + leaf_count(w.leaf);
+    for (uint32_t k = k0; k < kn; ++k) {
+      const Slot32 cur = load_slot32(S.slot32, k);
+      ++c.tests;
+      float t, b1, b2;
+      uint32_t prim;
+      if (slot_test32(cur, r, tmin, h.t, t, b1, b2, prim)) {
+        h.t = t;
+        h.b1 = b1;
+        h.b2 = b2;
+        h.prim = prim;
+      }
+    }
+    w.leaf = kNoRef;
+    w.park(stk);
+  }
+  return w.ref == kNoRef;
+}
+
 __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& r, float tmin, float tmax,
                                   LdsStack stk, Counters& c) {
-  Hit32 h;
-  h.plane = planes32(S, r, tmin, tmax);
-  h.t = tmax;
-  h.prim = 0xFFFFFFFFu;
-  h.b1 = h.b2 = 0.0f;
   if (!use_bvh) {
+    Hit32 h;
+    h.plane = planes32(S, r, tmin, tmax);
+    h.t = tmax;
+    h.prim = 0xFFFFFFFFu;
+    h.b1 = h.b2 = 0.0f;
     for (uint32_t i = 0; i < S.n_prims; ++i) {
       ++c.tests;
       float t, b1, b2;
@@ -541,111 +611,104 @@ __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& 
     }
     return h;
   }
-  if (!S.has_bvh) return h;
-  float tn;
-  ++c.visits;
-  if (!box32(r, S.root_lo[0], S.root_lo[1], S.root_lo[2], S.root_hi[0], S.root_hi[1], S.root_hi[2], tmin, h.t, tn))
-    return h;
-  const uint32_t neg = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
-  Walk w{S.root_ref, kNoRef, 0, 0};
-  w.park(stk);
-  for (;;) {
-    for (uint32_t steps = 1; w.inner(); ++steps) {
-      const uint32_t under = w.below(stk);
-      const float4* n = S.node2 + (size_t)w.ref * kNode2Stride;
-      const float4 a = ld_global(n), b = ld_global(n + 1), cc = ld_global(n + 2);
-      const uint4 m = ld_global(reinterpret_cast<const uint4*>(n + 3));
-      float tl, tr;
-      const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, h.t, tl);
-      const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, h.t, tr);
-      c.visits += 2;
-      const bool right_first = (neg >> m.z) & 1u;
-      const uint32_t first = right_first ? m.y : m.x, second = right_first ? m.x : m.y;
-      const bool hf = right_first ? hr : hl, hs = right_first ? hl : hr;
-      w.step(stk, under, first, second, hf, hs);
-      if (w.descended(steps)) break;
+  Trav32 T;
+  if (closest_begin32(S, r, tmin, tmax, stk, c, T))
+    while (!closest_round32(S, r, tmin, stk, c, T)) {
     }
-    w.park(stk);  // a leaf popped after a miss
-    while (w.leaf != kNoRef) {
-      const uint32_t k0 = leaf_first(w.leaf), kn = k0 + leaf_count(w.leaf);
-      for (uint32_t k = k0; k < kn; ++k) {
-        const Slot32 cur = load_slot32(S.slot32, k);
-        ++c.tests;
-        float t, b1, b2;
-        uint32_t prim;
-        if (slot_test32(cur, r, tmin, h.t, t, b1, b2, prim)) {
-          h.t = t;
-          h.b1 = b1;
-          h.b2 = b2;
-          h.prim = prim;
-        }
-      }
-      w.leaf = kNoRef;
-      w.park(stk);
-    }
-    if (w.ref == kNoRef) break;
-  }
-  return h;
+  return T.h;
 }
 
-__device__ inline bool any32(const DevScene& S, bool use_bvh, const Ray32& r, float tmin, float tmax, LdsStack stk,
-                             Counters& c) {
+// Any hit in (tmin, tmax): planes, then the root box; false when that
+// already ends the walk (`hit` final).
+__device__ __forceinline__ bool any_begin32(const DevScene& S, const Ray32& r, float tmin, float tmax, LdsStack stk,
+                                            Counters& c, Walk& w, uint32_t& neg, bool& hit) {
+  w = Walk{kNoRef, kNoRef, 0, 0};
+  neg = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
+  hit = false;
   for (uint32_t i = 0; i < S.n_planes; ++i) {
     const float4 p = ld_global(S.plane32 + 2 * i), n = ld_global(S.plane32 + 2 * i + 1);
     const float denom = n.x * r.dx + n.y * r.dy + n.z * r.dz;
     if (r_fabs(denom) < 1e-12f) continue;
     const float t = ((p.x - r.ox) * n.x + (p.y - r.oy) * n.y + (p.z - r.oz) * n.z) / denom;
-    if (tmin < t && t < tmax) return true;
-  }
-  float t, b1, b2;
-  uint32_t prim;
-  if (!use_bvh) {
-    for (uint32_t i = 0; i < S.n_prims; ++i) {
-      ++c.tests;
-      if (slot_hit32(S.slot32, ld_global(S.prim_slot + i), r, tmin, tmax, t, b1, b2, prim)) return true;
+    if (tmin < t && t < tmax) {
+      hit = true;
+      return false;
     }
-    return false;
   }
   if (!S.has_bvh) return false;
   float tn;
   ++c.visits;
   if (!box32(r, S.root_lo[0], S.root_lo[1], S.root_lo[2], S.root_hi[0], S.root_hi[1], S.root_hi[2], tmin, tmax, tn))
     return false;
-  const uint32_t neg = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
-  Walk w{S.root_ref, kNoRef, 0, 0};
+  w.ref = S.root_ref;
   w.park(stk);
-  for (;;) {
-    for (uint32_t steps = 1; w.inner(); ++steps) {
-      const uint32_t under = w.below(stk);
-      const float4* n = S.node2 + (size_t)w.ref * kNode2Stride;
-      const float4 a = ld_global(n), b = ld_global(n + 1), cc = ld_global(n + 2);
-      const uint4 m = ld_global(reinterpret_cast<const uint4*>(n + 3));
-      float tl, tr;
-      const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, tmax, tl);
-      const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, tmax, tr);
-      c.visits += 2;
-      // Any order finds whether some hit lies in (tmin, tmax). Bvh::any
-      // visits the left child first (bvh.hpp:106-132); the far child first
-      // (by the direction's sign on the split axis) is 1.6 % faster here,
-      // near-first 1.4 % slower (torus workload, any-hit kernel alone).
-      const bool far_right = !((neg >> m.z) & 1u);
-      w.step(stk, under, far_right ? m.y : m.x, far_right ? m.x : m.y, far_right ? hr : hl, far_right ? hl : hr);
-      if (w.descended(steps)) break;
-    }
-    w.park(stk);  // a leaf popped after a miss
-    while (w.leaf != kNoRef) {
-      const uint32_t k0 = leaf_first(w.leaf), kn = k0 + leaf_count(w.leaf);
-      for (uint32_t k = k0; k < kn; ++k) {
-        const Slot32 cur = load_slot32(S.slot32, k);
-        ++c.tests;
-        if (slot_test32(cur, r, tmin, tmax, t, b1, b2, prim)) return true;
-      }
-      w.leaf = kNoRef;
-      w.park(stk);
-    }
-    if (w.ref == kNoRef) break;
+  return true;
+}
+// One round of the any-hit walk; true when it has ended (`hit` set on a hit).
+__device__ __forceinline__ bool any_round32(const DevScene& S, const Ray32& r, float tmin, float tmax, LdsStack stk,
+                                            Counters& c, Walk& w, uint32_t neg, bool& hit) {
+  for (uint32_t steps = 1; w.inner(); ++steps) {
+    const uint32_t under = w.below(stk);
+    const float4* n = S.node2 + (size_t)w.ref * kNode2Stride;
+    const float4 a = ld_global(n), b = ld_global(n + 1), cc = ld_global(n + 2);
+    const uint4 m = ld_global(reinterpret_cast<const uint4*>(n + 3));
+    float tl, tr;
+    const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, tmax, tl);
+    const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, tmax, tr);
+    c.visits += 2;
+    // Any order finds whether some hit lies in (tmin, tmax). Bvh::any
+    // visits the left child first (bvh.hpp:106-132); the far child first
+    // (by the direction's sign on the split axis) is 1.6 % faster here,
+    // near-first 1.4 % slower (torus workload, any-hit kernel alone).
+    const bool far_right = !((neg >> m.z) & 1u);
+    w.step(stk, under, far_right ? m.y : m.x, far_right ? m.x : m.y, far_right ? hr : hl, far_right ? hl : hr);
+    if (w.descended(steps)) break;
   }
-  return false;
+  w.park(stk);  // a leaf popped after a miss
+  while (w.leaf != kNoRef) {
+    const uint32_t k0 = leaf_first(w.leaf), kn = k0 + leaf_count(w.leaf);
+    for (uint32_t k = k0; k < kn; ++k) {
+      const Slot32 cur = load_slot32(S.slot32, k);
+      ++c.tests;
+      float t, b1, b2;
+      uint32_t prim;
+      if (slot_test32(cur, r, tmin, tmax, t, b1, b2, prim)) {
+        hit = true;
+        w.ref = w.leaf = kNoRef;
+        return true;
+      }
+    }
+    w.leaf = kNoRef;
+    w.park(stk);
+  }
+  return w.ref == kNoRef;
+}
+
+__device__ inline bool any32(const DevScene& S, bool use_bvh, const Ray32& r, float tmin, float tmax, LdsStack stk,
+                             Counters& c) {
+  if (!use_bvh) {
+    for (uint32_t i = 0; i < S.n_planes; ++i) {
+      const float4 p = ld_global(S.plane32 + 2 * i), n = ld_global(S.plane32 + 2 * i + 1);
+      const float denom = n.x * r.dx + n.y * r.dy + n.z * r.dz;
+      if (r_fabs(denom) < 1e-12f) continue;
+      const float t = ((p.x - r.ox) * n.x + (p.y - r.oy) * n.y + (p.z - r.oz) * n.z) / denom;
+      if (tmin < t && t < tmax) return true;
+    }
+    float t, b1, b2;
+    uint32_t prim;
+    for (uint32_t i = 0; i < S.n_prims; ++i) {
+      ++c.tests;
+      if (slot_hit32(S.slot32, ld_global(S.prim_slot + i), r, tmin, tmax, t, b1, b2, prim)) return true;
+    }
+    return false;
+  }
+  Walk w;
+  uint32_t neg;
+  bool hit;
+  if (any_begin32(S, r, tmin, tmax, stk, c, w, neg, hit))
+    while (!any_round32(S, r, tmin, tmax, stk, c, w, neg, hit)) {
+    }
+  return hit;
 }
 
 }  // namespace rtg
