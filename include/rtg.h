@@ -265,11 +265,12 @@ rtg_status rtg_resolve(rtg_context* ctx, const rtg_accum* acc, int32_t tone_map,
 /* Batched rendering: many jobs — each one pass (samples [s0, s1) of pixels
  * [p0, p1)) of one render with its own scene, settings and accumulation
  * buffer — rendered together. FP32 path-integrator jobs with max_depth <= 64
- * and fewer than 4 Mi paths share the path arrays and every kernel launch
- * (planned on the device, so a batch of small renders fills the GPU like one
- * large one); other jobs, and jobs large enough to fill the GPU alone, run
- * through the single-render path in the same call. Every job's result is
- * bitwise the result of rendering it alone with rtg_render_pass.
+ * share the path arrays and every kernel launch (planned on the device, so a
+ * batch of small renders fills the GPU like one large one; groups of at
+ * least rtg_tuning.wide_paths paths run as wide passes); FP64, direct,
+ * normals and deeper jobs run through the single-render path in the same
+ * call. Every job's result is bitwise the result of rendering it alone with
+ * rtg_render_pass.
  * Jobs that write the same accumulation buffer must cover disjoint pixel
  * ranges. stats (n_jobs entries, may be NULL) receives each job's counters;
  * elapsed_ms is the device time of the whole call. The reference renders one

@@ -594,7 +594,12 @@ __device__ __forceinline__ void extend_range(const DevScene& S, const PathBuffer
       pid = active_pid(B, P, pre, b, i);
       r = make_ray32(B.ox[pid], B.oy[pid], B.oz[pid], B.dx[pid], B.dy[pid], B.dz[pid]);
       ++st[kStRays];
-      active = closest_begin32(S, r, 1e-6f, __builtin_inff(), stk, c, T);
+      if (P.use_bvh) {
+        active = closest_begin32(S, r, 1e-6f, __builtin_inff(), stk, c, T);
+      } else {  // brute force (Scene::intersect_brute): one pass over the primitives
+        T.h = closest32(S, false, r, 1e-6f, __builtin_inff(), stk, c);
+        active = false;
+      }
       if (!active) finish();
     }
     if (!__ballot(active)) {
@@ -637,7 +642,12 @@ __device__ __forceinline__ void shadow_range(const DevScene& S, const PathBuffer
       k = P.q_base + queue_slot(pre, P.sub_cap, i) + (uint32_t)(b & 1) * B.sq_stride;
       r = make_ray32(B.sox[k], B.soy[k], B.soz[k], B.sdx[k], B.sdy[k], B.sdz[k]);
       tmax = B.stm[k];
-      active = any_begin32(S, r, 1e-6f, tmax, stk, c, w, neg, hit);
+      if (P.use_bvh) {
+        active = any_begin32(S, r, 1e-6f, tmax, stk, c, w, neg, hit);
+      } else {
+        hit = any32(S, false, r, 1e-6f, tmax, stk, c);
+        active = false;
+      }
       if (!active) finish();
     }
     if (!__ballot(active)) {

@@ -146,3 +146,23 @@ def test_no_bvh_render_matches_bvh_render(ctx):
     b, _, _ = gpu_render(ctx, o2, 0, 4, rtg.PRECISION_F64)
     # scene.cpp:200-215 finds the same closest hits as the BVH
     assert np.abs(a - b).max() <= 1e-12
+
+
+@pytest.mark.parametrize("integrator", ["path", "direct"])
+def test_f32_no_bvh_render_is_brute_force(ctx, integrator):
+    """--no-bvh in FP32 (the persistent traversal kernels and the direct
+    queue): every closest-hit ray tests every primitive (scene.cpp:200-215)
+    and no node box, and the image equals the BVH render's bit for bit (the
+    FP32 walk's closest t equals brute force's; the sphere field has no
+    equal-t ties)."""
+    kw = {} if integrator == "path" else {"integrator": "direct"}
+    o = orc.OracleScene(scenes.load("sphere_field_c3.json"), width=48, height=27, **kw)
+    ob = orc.OracleScene(scenes.load("sphere_field_c3.json"), width=48, height=27, use_bvh=False, **kw)
+    a, ca, sta = gpu_render(ctx, o, 0, 2, rtg.PRECISION_F32)
+    b, cb, stb = gpu_render(ctx, ob, 0, 2, rtg.PRECISION_F32)
+    st_a, st_b = sta[0], stb[0]
+    n = ob.desc.n_prims
+    assert st_b["node_visits"] == 0
+    assert st_b["rays"] * n <= st_b["prim_tests"] <= (st_b["rays"] + st_b["shadow_rays"]) * n
+    assert (st_a["rays"], st_a["shadow_rays"]) == (st_b["rays"], st_b["shadow_rays"])
+    assert np.array_equal(ca, cb) and np.array_equal(a.view(np.uint64), b.view(np.uint64))
