@@ -148,7 +148,7 @@ def extend_roofline(prof, traffic, kind="extend"):
     # scaled by a copy of known size) over the same launch time.
     hbm = traffic / (avg_ms * 1e-3) / 1e9 if traffic and avg_ms > 0 else None
     label = ("k_direct (FP32 direct / normals sample, closest-hit and shadow traversal)" if kind == "direct"
-             else "k_extend (FP32 closest-hit traversal)")
+             else "k_extend_dyn (FP32 closest-hit traversal, persistent, lanes refilled)")
     return {"kernel": label, "bound": "hbm", "achieved": achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "hbm_achieved": hbm, "hbm_frac": hbm / HBM_PEAK_GBS if hbm is not None else None,
