@@ -1,6 +1,7 @@
 // rtg_f32.hip — FP32 instantiation of the path-tracing kernels (the
 // production path: float arithmetic, counter-hash RNG, LDS traversal stack).
 #include <algorithm>
+#include <cstdlib>
 
 #include "rtg_launch.hpp"
 
@@ -70,11 +71,17 @@ void chunk_impl(const DevScene& S, const PathBuffers<float>& B, const ChunkParam
     // for the last. Every path's additions keep their order.
     bool pending[2] = {false, false};
     // The traversal kernels are persistent (at most one device-full of
-    // workgroups, idle lanes refilled between rounds of their walks).
+    // workgroups, idle lanes refilled between rounds of their walks) unless
+    // RTG_PERSISTENT=0 (one ray per thread; the same results).
     static const uint32_t r_ext = resident_blocks(k_extend_dyn<STACK>), r_sh = resident_blocks(k_shadow_dyn<STACK>);
+    const char* pe = std::getenv("RTG_PERSISTENT");
+    const bool persistent = !(pe && pe[0] == '0');
     const uint32_t g_ext = std::min(g, r_ext), g_sh = std::min(g, r_sh);
     for (int b = 0; b < P.max_depth; ++b) {
-      RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend_dyn<STACK>), dim3(g_ext), dim3(kBlock), 0, s, S, B, P, b);
+      if (persistent)
+        RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend_dyn<STACK>), dim3(g_ext), dim3(kBlock), 0, s, S, B, P, b);
+      else
+        RTG_LAUNCH(h, RTG_K_EXTEND, (k_extend<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
       if (pending[b & 1]) hipStreamWaitEvent(s, side->shadowed[b & 1], 0);
       pending[b & 1] = false;
       RTG_LAUNCH(h, RTG_K_SHADE, k_shade<float>, dim3(g), dim3(kBlock), 0, s, S, B, P, b);
@@ -82,11 +89,16 @@ void chunk_impl(const DevScene& S, const PathBuffers<float>& B, const ChunkParam
       if (side) {
         hipEventRecord(side->shaded, s);
         hipStreamWaitEvent(side->stream, side->shaded, 0);
-        RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow_dyn<STACK>), dim3(g_sh), dim3(kBlock), 0, side->stream, S, B, P, b);
+        if (persistent)
+          RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow_dyn<STACK>), dim3(g_sh), dim3(kBlock), 0, side->stream, S, B, P, b);
+        else
+          RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<float, STACK>), dim3(g), dim3(kBlock), 0, side->stream, S, B, P, b);
         hipEventRecord(side->shadowed[b & 1], side->stream);
         pending[b & 1] = true;
-      } else {
+      } else if (persistent) {
         RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow_dyn<STACK>), dim3(g_sh), dim3(kBlock), 0, s, S, B, P, b);
+      } else {
+        RTG_LAUNCH(h, RTG_K_SHADOW, (k_shadow<float, STACK>), dim3(g), dim3(kBlock), 0, s, S, B, P, b);
       }
     }
     for (int p = 0; p < 2; ++p)

@@ -88,3 +88,26 @@ def test_passes_cancel_and_bad_bounds(ctx):
             ctx.render_passes(dev, cfg, bad, acc)
     acc.release()
     dev.release()
+
+
+@pytest.mark.parametrize("name,w,h,spp", [("torus_c2.json", 128, 96, 8), ("sphere_field_c3.json", 128, 72, 4),
+                                         ("cornell_c1.json", 96, 96, 8)])
+def test_persistent_traversal_equals_one_ray_per_thread(ctx, monkeypatch, name, w, h, spp):
+    """The persistent traversal kernels hand idle lanes new rays between
+    rounds of their walks; a ray's walk depends on the ray alone, so the
+    framebuffer and every counter equal those of one ray per thread
+    (RTG_PERSISTENT=0) bit for bit, also with chunks cut small."""
+    o = orc.OracleScene(scenes.load(name), width=w, height=h)
+    cfg = o.settings(rtg.PRECISION_F32)
+    dev = ctx.scene(o.desc)
+    bounds = schedule(spp)
+    monkeypatch.setenv("RTG_PERSISTENT", "0")
+    s0, c0, k0 = in_one_call(ctx, dev, cfg, bounds)
+    for env in ({"RTG_PERSISTENT": "1"}, {"RTG_PERSISTENT": "1", "RTG_CHUNK_PATHS": "4096"}):
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        s1, c1, k1 = in_one_call(ctx, dev, cfg, bounds)
+        assert np.array_equal(c1, c0), env
+        assert np.array_equal(s1.view(np.uint64), s0.view(np.uint64)), env
+        assert k1 == k0, env
+    dev.release()
