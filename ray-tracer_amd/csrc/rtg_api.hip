@@ -362,6 +362,10 @@ bool build_node_records(const rtg_scene_desc* d, std::vector<float4>& node2, uin
   }
   for (uint32_t i = 0; i < d->n_nodes; ++i)
     if (d->nodes[i].count == 0) n2_index[i] = (int32_t)n_interior++;
+  if (n_interior >= kMaxSlots) {  // node records are addressed by 32-bit byte offsets
+    err = "scene: more than 2^25 interior BVH nodes";
+    return false;
+  }
   auto child_ref = [&](uint32_t c) -> uint32_t {
     const rtg_bvh_node& n = d->nodes[c];
     return n.count > 0 ? leaf_ref(n.offset, n.count) : (uint32_t)n2_index[c];

@@ -577,7 +577,7 @@ __device__ __forceinline__ void extend_range(const DevScene& S, const PathBuffer
   const LdsStack stk = lds_stack(sref);
   Counters c{0, 0};
   Trav32 T;
-  T.w = Walk{kNoRef, kNoRef, 0, 0};
+  T.w = Walk{kNoRef, kNoRef, 0, kNoRef};
   Ray32 r{};
   uint32_t pid = 0;
   bool active = false, more = true;
@@ -623,7 +623,7 @@ __device__ __forceinline__ void shadow_range(const DevScene& S, const PathBuffer
                                              const Range& rg, uint32_t* st) {
   const LdsStack stk = lds_stack(sref);
   Counters c{0, 0};
-  Walk w{kNoRef, kNoRef, 0, 0};
+  Walk w{kNoRef, kNoRef, 0, kNoRef};
   Ray32 r{};
   uint32_t k = 0, neg = 0;
   float tmax = 0.0f;

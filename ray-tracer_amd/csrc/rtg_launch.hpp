@@ -9,14 +9,14 @@
 
 namespace rtg {
 
-// LDS stack classes of the FP32 traversal kernels (entries per lane). A
-// lane's stack top sits in a register and the LDS slots it uses lie below
-// the BVH depth, so a scene whose BVH depth + 1 is `stack` fits a class of
-// stack - 1 entries. 19 entries keep a traversal workgroup under 10 KB of
-// LDS: 16 workgroups (8 waves per SIMD) fit a CU.
+// LDS stack classes of the FP32 traversal kernels (slots per lane). A
+// walk whose inner nodes lie at depth d or less uses slots 0 .. d + 1, so a
+// scene whose BVH depth + 1 is `stack` (inner nodes at depth stack - 2 or
+// less) fits a class of `stack` slots. 19 slots keep a traversal workgroup
+// under 10 KB of LDS: 16 workgroups (8 waves per SIMD) fit a CU.
 constexpr uint32_t kStackClasses[3] = {19, 31, 63};
 inline int stack_class_of(uint32_t stack) {
-  const uint32_t need = stack > 1 ? stack - 1 : 1;
+  const uint32_t need = stack > 1 ? stack : 1;
   return need <= kStackClasses[0] ? 0 : need <= kStackClasses[1] ? 1 : 2;
 }
 

@@ -354,6 +354,20 @@ __device__ __forceinline__ uint4 ld_global(const uint4* p) {
   const NativeU4 v = *(const __attribute__((address_space(1))) NativeU4*)p;
   return uint4{v.x, v.y, v.z, v.w};
 }
+// The child refs and split axis of a node record (its fourth float4; the
+// last word is padding). The load still writes the padding word's register
+// when it returns; the compiler, seeing that word unused, may hand its
+// register to another value meanwhile and must then wait for the load before
+// writing it (measured: the traversal stack's LDS accesses waited for the
+// node's first load). The empty asm statement keeps the padding word live
+// until the node's boxes are tested.
+struct NodeRefs {
+  uint32_t x, y, z, pad;
+};
+__device__ __forceinline__ NodeRefs ld_node_refs(const float4* p) {
+  const uint4 v = ld_global(reinterpret_cast<const uint4*>(p));
+  return NodeRefs{v.x, v.y, v.z, v.w};
+}
 __device__ __forceinline__ uint32_t ld_global(const uint32_t* p) {
   return *(const __attribute__((address_space(1))) uint32_t*)p;
 }
@@ -368,7 +382,10 @@ struct Slot32 {
   float4 a, b, c;
 };
 __device__ __forceinline__ Slot32 load_slot32(const float4* __restrict__ slot32, uint32_t k) {
-  const float4* p = slot32 + (size_t)k * kSlot32Stride;
+  // A 32-bit byte offset (k < 2^25 slots of 48 bytes) from the array base:
+  // one address add per load group.
+  const float4* p = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(slot32) +
+                                                    k * (uint32_t)(kSlot32Stride * sizeof(float4)));
   const Slot32 s{ld_global(p), ld_global(p + 1), ld_global(p + 2)};
   asm volatile("" ::"v"(s.c.x));
   return s;
@@ -447,6 +464,14 @@ __device__ __forceinline__ int planes32(const DevScene& S, const Ray32& r, float
   return hit;
 }
 
+// Inner node record `ref` (< 2^25 records of 64 bytes): a 32-bit byte
+// offset from the array base.
+__device__ __forceinline__ const float4* node_at(const float4* __restrict__ node2, uint32_t ref) {
+  return reinterpret_cast<const float4*>(reinterpret_cast<const char*>(node2) +
+                                         (ref << 6));
+}
+static_assert(kNode2Stride * sizeof(float4) == 64, "two-child node records are 64 bytes");
+
 struct Hit32 {
   float t, b1, b2;
   uint32_t prim;  // 0xFFFFFFFF none
@@ -474,17 +499,25 @@ constexpr uint32_t kNoRef = 0xFFFFFFFFu;
 // own, images and counters are independent of chunking, batching and queue
 // order.
 constexpr uint32_t kDescent = 24;
+// Stack layout: entry i (1-based, i < n) of a walk lives in LDS slot i + 1,
+// and a step spills the top to slot n + 1, so slot 1 holds kNoRef whenever
+// n is 1 (the step that pushed from an empty stack spilled its kNoRef top
+// there) and slot 0 is read only by an empty stack, whose pop ends the walk.
+// The slot addresses are then n and n + 1 with no clamping (a walk whose
+// inner nodes lie at depth d or less uses slots 0 .. d + 1). The top is
+// kNoRef exactly when the stack is empty.
 struct Walk {
   uint32_t ref;   // next node to visit (kNoRef: traversal done)
   uint32_t leaf;  // parked leaf (kNoRef: none)
-  uint32_t n;     // stack entries: the top in `top`, the rest in LDS slots 0 .. n-2
-  uint32_t top;
+  uint32_t n;     // stack entries: the top in `top`, the rest in LDS slots 2 .. n
+  uint32_t top;   // kNoRef when n == 0
 
   __device__ __forceinline__ bool inner() const { return !(ref & kLeafBit); }
-  // Entry n-2, the top after a pop (any slot's contents when n < 2).
-  __device__ __forceinline__ uint32_t below(LdsStack s) const { return s.ref[(n >= 2 ? n - 2 : 0) * kStackStride]; }
+  // Entry n-1, the top after a pop (kNoRef when n == 1; slot 0's contents
+  // when n == 0).
+  __device__ __forceinline__ uint32_t below(LdsStack s) const { return s.ref[n * kStackStride]; }
   __device__ __forceinline__ uint32_t pop(uint32_t under) {
-    const uint32_t r = n ? top : kNoRef;
+    const uint32_t r = top;
     top = under;
     n -= n ? 1u : 0u;
     return r;
@@ -496,23 +529,24 @@ struct Walk {
       ref = pop(below(s));
     }
   }
-  // One inner node with children `first` (visited first) and `second` and
-  // whether their boxes were hit; `under` = below(), read before the node.
-  // Branch-free: the top is spilled to its slot on every step (kept only by
-  // a push; at an inner node of depth d the stack holds at most d entries,
-  // so the slot lies below the BVH depth), and a child leaf is parked when
-  // none is, continuing with the other child or the stack.
-  __device__ __forceinline__ void step(LdsStack s, uint32_t under, uint32_t first, uint32_t second, bool hf,
-                                       bool hs) {
-    s.ref[(n ? n - 1 : 0) * kStackStride] = top;
-    const uint32_t c = hf ? first : second;
-    const bool both = hf && hs, any = hf || hs;
+  // One inner node with children l and r, whether their boxes were hit,
+  // and whether r is visited first when both are; `under` = below(), read
+  // before the node. Branch-free: the top is spilled to its slot on every
+  // step (kept only by a push; at an inner node of depth d the stack holds
+  // at most d entries, so the slot is at most d + 1), and a child leaf is
+  // parked when none is, continuing with the other child or the stack. The
+  // child taken is the first when both boxes were hit, else the one hit.
+  __device__ __forceinline__ void step(LdsStack s, uint32_t under, uint32_t l, uint32_t r, bool hl, bool hr,
+                                       bool r_first) {
+    s.ref[(n + 1) * kStackStride] = top;
+    const bool both = hl && hr, any = hl || hr;
+    const uint32_t first = r_first ? r : l, second = r_first ? l : r;
+    const uint32_t c = both ? first : (hl ? l : r);
     const bool park_c = any && leaf == kNoRef && (c & kLeafBit);
     const bool push = both && !park_c;
     const bool popping = !any || (park_c && !both);
-    const uint32_t popped = n ? top : kNoRef;
     leaf = park_c ? c : leaf;
-    ref = popping ? popped : (park_c ? second : c);
+    ref = popping ? top : (park_c ? second : c);
     top = push ? second : popping ? under : top;
     n = push ? n + 1 : (popping && n) ? n - 1 : n;
   }
@@ -537,7 +571,7 @@ __device__ __forceinline__ bool closest_begin32(const DevScene& S, const Ray32& 
   T.h.t = tmax;
   T.h.prim = 0xFFFFFFFFu;
   T.h.b1 = T.h.b2 = 0.0f;
-  T.w = Walk{kNoRef, kNoRef, 0, 0};
+  T.w = Walk{kNoRef, kNoRef, 0, kNoRef};
   T.neg = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
   if (!S.has_bvh) return false;
   float tn;
@@ -556,17 +590,15 @@ __device__ __forceinline__ bool closest_round32(const DevScene& S, const Ray32& 
   Hit32& h = T.h;
   for (uint32_t steps = 1; w.inner(); ++steps) {
     const uint32_t under = w.below(stk);
-    const float4* n = S.node2 + (size_t)w.ref * kNode2Stride;
+    const float4* n = node_at(S.node2, w.ref);
     const float4 a = ld_global(n), b = ld_global(n + 1), cc = ld_global(n + 2);
-    const uint4 m = ld_global(reinterpret_cast<const uint4*>(n + 3));
+    const NodeRefs m = ld_node_refs(n + 3);
     float tl, tr;
     const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, h.t, tl);
     const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, h.t, tr);
     c.visits += 2;
-    const bool right_first = (T.neg >> m.z) & 1u;
-    const uint32_t first = right_first ? m.y : m.x, second = right_first ? m.x : m.y;
-    const bool hf = right_first ? hr : hl, hs = right_first ? hl : hr;
-    w.step(stk, under, first, second, hf, hs);
+    asm volatile("" ::"v"(m.pad));
+    w.step(stk, under, m.x, m.y, hl, hr, (T.neg >> m.z) & 1u);
     if (w.descended(steps)) break;
   }
   w.park(stk);  // a leaf popped after a miss
@@ -622,7 +654,7 @@ __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& 
 // already ends the walk (`hit` final).
 __device__ __forceinline__ bool any_begin32(const DevScene& S, const Ray32& r, float tmin, float tmax, LdsStack stk,
                                             Counters& c, Walk& w, uint32_t& neg, bool& hit) {
-  w = Walk{kNoRef, kNoRef, 0, 0};
+  w = Walk{kNoRef, kNoRef, 0, kNoRef};
   neg = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
   hit = false;
   for (uint32_t i = 0; i < S.n_planes; ++i) {
@@ -649,9 +681,9 @@ __device__ __forceinline__ bool any_round32(const DevScene& S, const Ray32& r, f
                                             Counters& c, Walk& w, uint32_t neg, bool& hit) {
   for (uint32_t steps = 1; w.inner(); ++steps) {
     const uint32_t under = w.below(stk);
-    const float4* n = S.node2 + (size_t)w.ref * kNode2Stride;
+    const float4* n = node_at(S.node2, w.ref);
     const float4 a = ld_global(n), b = ld_global(n + 1), cc = ld_global(n + 2);
-    const uint4 m = ld_global(reinterpret_cast<const uint4*>(n + 3));
+    const NodeRefs m = ld_node_refs(n + 3);
     float tl, tr;
     const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, tmax, tl);
     const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, tmax, tr);
@@ -660,8 +692,8 @@ __device__ __forceinline__ bool any_round32(const DevScene& S, const Ray32& r, f
     // visits the left child first (bvh.hpp:106-132); the far child first
     // (by the direction's sign on the split axis) is 1.6 % faster here,
     // near-first 1.4 % slower (torus workload, any-hit kernel alone).
-    const bool far_right = !((neg >> m.z) & 1u);
-    w.step(stk, under, far_right ? m.y : m.x, far_right ? m.x : m.y, far_right ? hr : hl, far_right ? hl : hr);
+    asm volatile("" ::"v"(m.pad));
+    w.step(stk, under, m.x, m.y, hl, hr, !((neg >> m.z) & 1u));
     if (w.descended(steps)) break;
   }
   w.park(stk);  // a leaf popped after a miss
