@@ -416,22 +416,30 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
   if (i < qv.total) {
     pid = active_pid(B, P, pre, b, i);
     const uint32_t pixel = pid_pixel(P, pid), sample = pid_sample(P, pid);
-    const V3<T> o{B.ox[pid], B.oy[pid], B.oz[pid]}, d{B.dx[pid], B.dy[pid], B.dz[pid]};
+    const V3<T> d{B.dx[pid], B.dy[pid], B.dz[pid]};
     V3<T> beta{B.bx[pid], B.by[pid], B.bz[pid]};
-    V3<T> L{B.lx[pid], B.ly[pid], B.lz[pid]};
-    const bool spec = B.spec[pid] != 0;
     const uint32_t prim = B.hprim[pid];
     const int plane = B.hplane[pid];
+    // The path's radiance is read and written only by a bounce that adds
+    // to it (a miss, or emission counted here); most bounces add nothing.
     if (prim == 0xFFFFFFFFu && plane < 0) {
+      V3<T> L{B.lx[pid], B.ly[pid], B.lz[pid]};
       L = L + beta * background<T>(S, d);
-    } else {
+      B.lx[pid] = L.x; B.ly[pid] = L.y; B.lz[pid] = L.z;
+    } else {  // the origin and the specular flag are read by hits only
+      const V3<T> o{B.ox[pid], B.oy[pid], B.oz[pid]};
+      const bool spec = B.spec[pid] != 0;
       auto rng = DrawSrc<T>::make(B, P, pid, pixel, sample, (uint32_t)b);
       HitRec<T> rec;
       ModeOps<T>::finalize(S, o, d, B.ht[pid], B.hb1[pid], B.hb2[pid], prim, plane, rec);
       const Mat<T> mat = load_mat<T>(S, rec.material);
       const V3<T> Le = emitted(mat, rec.front_face);
       const bool sampled = !rec.is_plane && S.sampled[rec.prim] != 0;
-      if (!is_zero(Le) && (spec || !P.light_sampling || !sampled)) L = L + beta * Le;
+      if (!is_zero(Le) && (spec || !P.light_sampling || !sampled)) {
+        V3<T> L{B.lx[pid], B.ly[pid], B.lz[pid]};
+        L = L + beta * Le;
+        B.lx[pid] = L.x; B.ly[pid] = L.y; B.lz[pid] = L.z;
+      }
       Scatter<T> sr;
       if (scatter(mat, d, rec, rng, sr)) {
         if (P.light_sampling && !sr.specular && S.n_lights > 0) {
@@ -466,7 +474,6 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
       }
       DrawSrc<T>::store(B, pid, rng);
     }
-    B.lx[pid] = L.x; B.ly[pid] = L.y; B.lz[pid] = L.z;
   }
   // Converged: compacted appends to the next active queue and the shadow queue.
   const AppendSlots a = block_append2(P.counts + (size_t)(b + 1) * kSubQueues * kCountStride,
