@@ -372,6 +372,20 @@ __device__ __forceinline__ uint32_t active_pid(const PathBuffers<T>& B, const Ch
 }
 
 // extend: closest hit for the active queue of bounce b.
+// A closest hit for the shade kernel, which reads the distance of hits only
+// and the barycentrics of primitive hits only: a miss stores its ids, a
+// plane hit its ids and distance.
+template <class T, class H>
+__device__ __forceinline__ void store_hit(const PathBuffers<T>& B, uint32_t pid, const H& h) {
+  B.hprim[pid] = h.prim;
+  B.hplane[pid] = h.plane;
+  if (h.prim != 0xFFFFFFFFu || h.plane >= 0) B.ht[pid] = h.t;
+  if (h.prim != 0xFFFFFFFFu) {
+    B.hb1[pid] = h.b1;
+    B.hb2[pid] = h.b2;
+  }
+}
+
 template <class T, int STACK>
 __device__ __forceinline__ void extend_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P, int b,
                                             uint32_t lb, uint32_t* sref, uint32_t* pre) {
@@ -384,11 +398,7 @@ __device__ __forceinline__ void extend_body(const DevScene& S, const PathBuffers
     const V3<T> o{B.ox[pid], B.oy[pid], B.oz[pid]}, d{B.dx[pid], B.dy[pid], B.dz[pid]};
     Counters c{0, 0};
     const auto h = ModeOps<T>::closest(S, P.use_bvh != 0, o, d, lds_stack(sref), c);
-    B.ht[pid] = h.t;
-    B.hb1[pid] = h.b1;
-    B.hb2[pid] = h.b2;
-    B.hprim[pid] = h.prim;
-    B.hplane[pid] = h.plane;
+    store_hit(B, pid, h);
     st[kStRays] = 1;
     st[kStVisits] = c.visits;
     st[kStTests] = c.tests;
@@ -588,13 +598,7 @@ __device__ __forceinline__ void extend_range(const DevScene& S, const PathBuffer
   Ray32 r{};
   uint32_t pid = 0;
   bool active = false, more = true;
-  auto finish = [&]() {
-    B.ht[pid] = T.h.t;
-    B.hb1[pid] = T.h.b1;
-    B.hb2[pid] = T.h.b2;
-    B.hprim[pid] = T.h.prim;
-    B.hplane[pid] = T.h.plane;
-  };
+  auto finish = [&]() { store_hit(B, pid, T.h); };
   for (;;) {
     uint32_t i = 0;
     if (refill(cursor, rg, __ballot(!active), more, i) && !active && i < rg.r1) {
