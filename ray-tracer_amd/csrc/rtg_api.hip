@@ -820,7 +820,7 @@ uint64_t lay_out_paths(char* base, uint32_t cap, bool batch, unsigned long long*
   const uint32_t sub_cap = sub_queue_cap(cap);
   const uint64_t c = batch ? (uint64_t)cap + kBatchQueueSlack : (uint64_t)sub_cap * kSubQueues;  // >= cap
   auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
-  // 6 ray + 3 beta + 3 L + 3 hit + 10 shadow arrays of T; 2 RNG u64; 5 u32
+  // 12 ray (two buffers) + 3 beta + 3 L + 3 hit + 10 shadow arrays of T; 2 RNG u64; 5 u32
   // (2 queues, hprim, hplane, sq_pid); spec; counts. FP32: a second shadow
   // queue (10 T + sq_pid) and 3 shadow-radiance arrays.
   constexpr bool f32 = sizeof(T) == 4;
@@ -828,7 +828,7 @@ uint64_t lay_out_paths(char* base, uint32_t cap, bool batch, unsigned long long*
   const uint64_t n_counts =
       (batch ? (uint64_t)kMaxBatchJobs * kBatchCounts : 2ull * kMaxBounce * kSubQueues) * kCountStride;
   const uint64_t n_stats = batch ? (uint64_t)kMaxBatchJobs * kBatchStatSlots * kStatStride : 0;
-  const uint64_t bytes = al(c * sizeof(T)) * (15 + (f32 ? 3 : 0)) + al(sq * sizeof(T)) * 10 + al(c * 8) * 2 +
+  const uint64_t bytes = al(c * sizeof(T)) * (21 + (f32 ? 3 : 0)) + al(sq * sizeof(T)) * 10 + al(c * 8) * 2 +
                          al(c * 4) * 4 + al(sq * 4) + al(c) + al(sizeof(uint32_t) * n_counts) + al(8 * n_stats);
   if (!base) return bytes;
   char* p = base;
@@ -837,8 +837,10 @@ uint64_t lay_out_paths(char* base, uint32_t cap, bool batch, unsigned long long*
     p += al(b);
     return r;
   };
-  T** tarr[] = {&B->ox, &B->oy, &B->oz, &B->dx, &B->dy, &B->dz, &B->bx, &B->by, &B->bz, &B->lx, &B->ly, &B->lz,
-                &B->ht, &B->hb1, &B->hb2};
+  T** tarr[] = {&B->ray[0].ox, &B->ray[0].oy, &B->ray[0].oz, &B->ray[0].dx, &B->ray[0].dy, &B->ray[0].dz,
+                &B->ray[1].ox, &B->ray[1].oy, &B->ray[1].oz, &B->ray[1].dx, &B->ray[1].dy, &B->ray[1].dz,
+                &B->bx,        &B->by,        &B->bz,        &B->lx,        &B->ly,        &B->lz,
+                &B->ht,        &B->hb1,       &B->hb2};
   for (T** a : tarr) *a = reinterpret_cast<T*>(take(c * sizeof(T)));
   T** sarr[] = {&B->sox, &B->soy, &B->soz, &B->sdx, &B->sdy, &B->sdz, &B->stm, &B->scx, &B->scy, &B->scz};
   for (T** a : sarr) *a = reinterpret_cast<T*>(take(sq * sizeof(T)));

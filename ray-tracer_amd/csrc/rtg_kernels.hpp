@@ -31,14 +31,22 @@ enum StatIdx {
 };
 
 template <class T>
+struct RayArrays {
+  T *ox, *oy, *oz, *dx, *dy, *dz;
+};
+template <class T>
 struct PathBuffers {
-  T *ox, *oy, *oz, *dx, *dy, *dz;  // current ray
+  // Rays of bounce b in ray[b & 1], by queue slot (bounce 0: by path id),
+  // and the closest hits of bounce b by the same slot: the closest-hit and
+  // shade kernels read and write them with the queue's own (coalesced)
+  // order, and the shade kernel reads them without waiting for the path id.
+  RayArrays<T> ray[2];
   T *bx, *by, *bz;                 // throughput
   T *lx, *ly, *lz;                 // radiance
   uint8_t* spec;                   // specular_bounce
-  T *ht, *hb1, *hb2;               // hit t, barycentrics
-  uint32_t* hprim;                 // hit primitive (0xFFFFFFFF none)
-  int32_t* hplane;                 // hit plane (-1 none)
+  T *ht, *hb1, *hb2;               // hit t, barycentrics (by queue slot)
+  uint32_t* hprim;                 // hit primitive (0xFFFFFFFF none; by queue slot)
+  int32_t* hplane;                 // hit plane (-1 none; by queue slot)
   uint64_t *rs, *ri;               // PCG32 state/inc (FP64 mode)
   uint32_t* q[2];                  // active queues (kSubQueues sub-queues of sub_cap each)
   uint32_t* counts;                // sub-queue sizes (x kCountStride): [b*kSubQueues + s] active, [(kMaxBounce+b)*kSubQueues + s] shadow
@@ -350,8 +358,9 @@ __device__ __forceinline__ void generate_body(const DevScene& S, const PathBuffe
     const float jy = rng.next();
     ModeOps<T>::camera(S, x, y, jx, jy, rng, o, d);
   }
-  B.ox[pid] = o.x; B.oy[pid] = o.y; B.oz[pid] = o.z;
-  B.dx[pid] = d.x; B.dy[pid] = d.y; B.dz[pid] = d.z;
+  const RayArrays<T>& R = B.ray[0];
+  R.ox[pid] = o.x; R.oy[pid] = o.y; R.oz[pid] = o.z;
+  R.dx[pid] = d.x; R.dy[pid] = d.y; R.dz[pid] = d.z;
   B.bx[pid] = 1; B.by[pid] = 1; B.bz[pid] = 1;
   B.lx[pid] = 0; B.ly[pid] = 0; B.lz[pid] = 0;
   if constexpr (sizeof(T) == 4) {
@@ -364,28 +373,31 @@ __global__ __launch_bounds__(kBlock) void k_generate(DevScene S, PathBuffers<T> 
   generate_body<T>(S, B, P, blockIdx.x);
 }
 
-// Path id of active entry i at bounce b (identity at bounce 0).
+// Queue slot of active entry i at bounce b (its path id at bounce 0, whose
+// queue is the identity and is not stored), and the entry's path id.
+__device__ __forceinline__ uint32_t active_slot(const ChunkParams& P, const uint32_t* pre, int b, uint32_t i) {
+  return b == 0 ? P.pid_base + i : P.q_base + queue_slot(pre, P.sub_cap, i);
+}
 template <class T>
-__device__ __forceinline__ uint32_t active_pid(const PathBuffers<T>& B, const ChunkParams& P, const uint32_t* pre,
-                                               int b, uint32_t i) {
-  return b == 0 ? P.pid_base + i : B.q[b & 1][P.q_base + queue_slot(pre, P.sub_cap, i)];
+__device__ __forceinline__ uint32_t slot_pid(const PathBuffers<T>& B, int b, uint32_t slot) {
+  return b == 0 ? slot : B.q[b & 1][slot];
 }
 
-// extend: closest hit for the active queue of bounce b.
 // A closest hit for the shade kernel, which reads the distance of hits only
 // and the barycentrics of primitive hits only: a miss stores its ids, a
 // plane hit its ids and distance.
 template <class T, class H>
-__device__ __forceinline__ void store_hit(const PathBuffers<T>& B, uint32_t pid, const H& h) {
-  B.hprim[pid] = h.prim;
-  B.hplane[pid] = h.plane;
-  if (h.prim != 0xFFFFFFFFu || h.plane >= 0) B.ht[pid] = h.t;
+__device__ __forceinline__ void store_hit(const PathBuffers<T>& B, uint32_t k, const H& h) {
+  B.hprim[k] = h.prim;
+  B.hplane[k] = h.plane;
+  if (h.prim != 0xFFFFFFFFu || h.plane >= 0) B.ht[k] = h.t;
   if (h.prim != 0xFFFFFFFFu) {
-    B.hb1[pid] = h.b1;
-    B.hb2[pid] = h.b2;
+    B.hb1[k] = h.b1;
+    B.hb2[k] = h.b2;
   }
 }
 
+// extend: closest hit for the active queue of bounce b.
 template <class T, int STACK>
 __device__ __forceinline__ void extend_body(const DevScene& S, const PathBuffers<T>& B, const ChunkParams& P, int b,
                                             uint32_t lb, uint32_t* sref, uint32_t* pre) {
@@ -394,11 +406,12 @@ __device__ __forceinline__ void extend_body(const DevScene& S, const PathBuffers
   const uint32_t i = lb * kBlock + threadIdx.x;
   uint32_t st[kStCount] = {0};
   if (i < qv.total) {
-    const uint32_t pid = active_pid(B, P, pre, b, i);
-    const V3<T> o{B.ox[pid], B.oy[pid], B.oz[pid]}, d{B.dx[pid], B.dy[pid], B.dz[pid]};
+    const uint32_t k = active_slot(P, pre, b, i);
+    const RayArrays<T>& R = B.ray[b & 1];
+    const V3<T> o{R.ox[k], R.oy[k], R.oz[k]}, d{R.dx[k], R.dy[k], R.dz[k]};
     Counters c{0, 0};
     const auto h = ModeOps<T>::closest(S, P.use_bvh != 0, o, d, lds_stack(sref), c);
-    store_hit(B, pid, h);
+    store_hit(B, k, h);
     st[kStRays] = 1;
     st[kStVisits] = c.visits;
     st[kStTests] = c.tests;
@@ -423,13 +436,16 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
   bool alive = false, shadow = false;
   uint32_t pid = 0, shadow_cls = 0;
   ShadowReq<T> req;
+  V3<T> nro, ndir;  // the next ray of a path that lives on
   if (i < qv.total) {
-    pid = active_pid(B, P, pre, b, i);
+    const uint32_t k = active_slot(P, pre, b, i);
+    const RayArrays<T>& R = B.ray[b & 1];
+    const V3<T> d{R.dx[k], R.dy[k], R.dz[k]};
+    const uint32_t prim = B.hprim[k];
+    const int plane = B.hplane[k];
+    pid = slot_pid(B, b, k);
     const uint32_t pixel = pid_pixel(P, pid), sample = pid_sample(P, pid);
-    const V3<T> d{B.dx[pid], B.dy[pid], B.dz[pid]};
     V3<T> beta{B.bx[pid], B.by[pid], B.bz[pid]};
-    const uint32_t prim = B.hprim[pid];
-    const int plane = B.hplane[pid];
     // The path's radiance is read and written only by a bounce that adds
     // to it (a miss, or emission counted here); most bounces add nothing.
     if (prim == 0xFFFFFFFFu && plane < 0) {
@@ -437,11 +453,11 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
       L = L + beta * background<T>(S, d);
       B.lx[pid] = L.x; B.ly[pid] = L.y; B.lz[pid] = L.z;
     } else {  // the origin and the specular flag are read by hits only
-      const V3<T> o{B.ox[pid], B.oy[pid], B.oz[pid]};
+      const V3<T> o{R.ox[k], R.oy[k], R.oz[k]};
       const bool spec = B.spec[pid] != 0;
       auto rng = DrawSrc<T>::make(B, P, pid, pixel, sample, (uint32_t)b);
       HitRec<T> rec;
-      ModeOps<T>::finalize(S, o, d, B.ht[pid], B.hb1[pid], B.hb2[pid], prim, plane, rec);
+      ModeOps<T>::finalize(S, o, d, B.ht[k], B.hb1[k], B.hb2[k], prim, plane, rec);
       const Mat<T> mat = load_mat<T>(S, rec.material);
       const V3<T> Le = emitted(mat, rec.front_face);
       const bool sampled = !rec.is_plane && S.sampled[rec.prim] != 0;
@@ -474,12 +490,11 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
           }
         }
         if (live && b + 1 < P.max_depth) {
-          const V3<T> nro = offset_ray_origin(rec.p, rec.gn, sr.dir);
+          nro = offset_ray_origin(rec.p, rec.gn, sr.dir);
+          ndir = sr.dir;
           alive = true;
           B.bx[pid] = beta.x; B.by[pid] = beta.y; B.bz[pid] = beta.z;
           B.spec[pid] = sr.specular ? 1 : 0;
-          B.ox[pid] = nro.x; B.oy[pid] = nro.y; B.oz[pid] = nro.z;
-          B.dx[pid] = sr.dir.x; B.dy[pid] = sr.dir.y; B.dz[pid] = sr.dir.z;
         }
       }
       DrawSrc<T>::store(B, pid, rng);
@@ -490,7 +505,13 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
                                       P.counts + (size_t)(P.shadow_counts + b * kSubQueues) * kCountStride, P.sub_cap,
                                       alive, shadow,
                                       shadow_cls, lb);
-  if (alive) B.q[(b + 1) & 1][P.q_base + a.a] = pid;
+  if (alive) {
+    const uint32_t k = P.q_base + a.a;
+    const RayArrays<T>& R = B.ray[(b + 1) & 1];
+    B.q[(b + 1) & 1][k] = pid;
+    R.ox[k] = nro.x; R.oy[k] = nro.y; R.oz[k] = nro.z;
+    R.dx[k] = ndir.x; R.dy[k] = ndir.y; R.dz[k] = ndir.z;
+  }
   if (shadow) {
     const int64_t si = P.q_base + a.b + (int64_t)(b & 1) * B.sq_stride;
     B.sq_pid[si] = pid;
@@ -596,14 +617,15 @@ __device__ __forceinline__ void extend_range(const DevScene& S, const PathBuffer
   Trav32 T;
   T.w = Walk{kNoRef, kNoRef, 0, kNoRef};
   Ray32 r{};
-  uint32_t pid = 0;
+  uint32_t k = 0;  // the ray's queue slot
   bool active = false, more = true;
-  auto finish = [&]() { store_hit(B, pid, T.h); };
+  auto finish = [&]() { store_hit(B, k, T.h); };
   for (;;) {
     uint32_t i = 0;
     if (refill(cursor, rg, __ballot(!active), more, i) && !active && i < rg.r1) {
-      pid = active_pid(B, P, pre, b, i);
-      r = make_ray32(B.ox[pid], B.oy[pid], B.oz[pid], B.dx[pid], B.dy[pid], B.dz[pid]);
+      k = active_slot(P, pre, b, i);
+      const RayArrays<float>& R = B.ray[b & 1];
+      r = make_ray32(R.ox[k], R.oy[k], R.oz[k], R.dx[k], R.dy[k], R.dz[k]);
       ++st[kStRays];
       if (P.use_bvh) {
         active = closest_begin32(S, r, 1e-6f, __builtin_inff(), stk, c, T);
