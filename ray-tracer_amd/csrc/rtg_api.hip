@@ -820,16 +820,17 @@ uint64_t lay_out_paths(char* base, uint32_t cap, bool batch, unsigned long long*
   const uint32_t sub_cap = sub_queue_cap(cap);
   const uint64_t c = batch ? (uint64_t)cap + kBatchQueueSlack : (uint64_t)sub_cap * kSubQueues;  // >= cap
   auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
-  // 12 ray (two buffers) + 3 beta + 3 L + 3 hit + 10 shadow arrays of T; 2 RNG u64; 5 u32
-  // (2 queues, hprim, hplane, sq_pid); spec; counts. FP32: a second shadow
-  // queue (10 T + sq_pid) and 3 shadow-radiance arrays.
+  // two buffers of 6 ray + 3 throughput arrays of T and a specular flag;
+  // 3 L + 3 hit + 10 shadow arrays of T; 2 RNG u64; 5 u32 (2 queues, hprim,
+  // hplane, sq_pid); counts. FP32: a second shadow queue (10 T + sq_pid) and
+  // 3 shadow-radiance arrays.
   constexpr bool f32 = sizeof(T) == 4;
   const uint64_t sq = f32 ? 2 * c : c;  // shadow-queue entries
   const uint64_t n_counts =
       (batch ? (uint64_t)kMaxBatchJobs * kBatchCounts : 2ull * kMaxBounce * kSubQueues) * kCountStride;
   const uint64_t n_stats = batch ? (uint64_t)kMaxBatchJobs * kBatchStatSlots * kStatStride : 0;
-  const uint64_t bytes = al(c * sizeof(T)) * (21 + (f32 ? 3 : 0)) + al(sq * sizeof(T)) * 10 + al(c * 8) * 2 +
-                         al(c * 4) * 4 + al(sq * 4) + al(c) + al(sizeof(uint32_t) * n_counts) + al(8 * n_stats);
+  const uint64_t bytes = al(c * sizeof(T)) * (24 + (f32 ? 3 : 0)) + al(sq * sizeof(T)) * 10 + al(c * 8) * 2 +
+                         al(c * 4) * 4 + al(sq * 4) + al(c) * 2 + al(sizeof(uint32_t) * n_counts) + al(8 * n_stats);
   if (!base) return bytes;
   char* p = base;
   auto take = [&](uint64_t b) {
@@ -838,9 +839,9 @@ uint64_t lay_out_paths(char* base, uint32_t cap, bool batch, unsigned long long*
     return r;
   };
   T** tarr[] = {&B->ray[0].ox, &B->ray[0].oy, &B->ray[0].oz, &B->ray[0].dx, &B->ray[0].dy, &B->ray[0].dz,
-                &B->ray[1].ox, &B->ray[1].oy, &B->ray[1].oz, &B->ray[1].dx, &B->ray[1].dy, &B->ray[1].dz,
-                &B->bx,        &B->by,        &B->bz,        &B->lx,        &B->ly,        &B->lz,
-                &B->ht,        &B->hb1,       &B->hb2};
+                &B->ray[0].bx, &B->ray[0].by, &B->ray[0].bz, &B->ray[1].ox, &B->ray[1].oy, &B->ray[1].oz,
+                &B->ray[1].dx, &B->ray[1].dy, &B->ray[1].dz, &B->ray[1].bx, &B->ray[1].by, &B->ray[1].bz,
+                &B->lx,        &B->ly,        &B->lz,        &B->ht,        &B->hb1,       &B->hb2};
   for (T** a : tarr) *a = reinterpret_cast<T*>(take(c * sizeof(T)));
   T** sarr[] = {&B->sox, &B->soy, &B->soz, &B->sdx, &B->sdy, &B->sdz, &B->stm, &B->scx, &B->scy, &B->scz};
   for (T** a : sarr) *a = reinterpret_cast<T*>(take(sq * sizeof(T)));
@@ -858,7 +859,8 @@ uint64_t lay_out_paths(char* base, uint32_t cap, bool batch, unsigned long long*
   B->hprim = reinterpret_cast<uint32_t*>(take(c * 4));
   B->hplane = reinterpret_cast<int32_t*>(take(c * 4));
   B->sq_pid = reinterpret_cast<uint32_t*>(take(sq * 4));
-  B->spec = reinterpret_cast<uint8_t*>(take(c));
+  B->ray[0].spec = reinterpret_cast<uint8_t*>(take(c));
+  B->ray[1].spec = reinterpret_cast<uint8_t*>(take(c));
   B->counts = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * n_counts));
   B->stats = batch ? reinterpret_cast<unsigned long long*>(take(8 * n_stats)) : stats;
   B->capacity = cap;

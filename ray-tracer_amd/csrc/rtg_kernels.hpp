@@ -32,18 +32,20 @@ enum StatIdx {
 
 template <class T>
 struct RayArrays {
-  T *ox, *oy, *oz, *dx, *dy, *dz;
+  T *ox, *oy, *oz, *dx, *dy, *dz;  // the ray
+  T *bx, *by, *bz;                 // throughput
+  uint8_t* spec;                   // specular_bounce
 };
 template <class T>
 struct PathBuffers {
-  // Rays of bounce b in ray[b & 1], by queue slot (bounce 0: by path id),
-  // and the closest hits of bounce b by the same slot: the closest-hit and
-  // shade kernels read and write them with the queue's own (coalesced)
-  // order, and the shade kernel reads them without waiting for the path id.
+  // Rays of bounce b with their paths' throughput and specular flag in
+  // ray[b & 1], by queue slot (bounce 0: rays by path id; throughput 1 and
+  // the flag set for every path, not stored), and the closest hits of bounce
+  // b by the same slot: the closest-hit and shade kernels read and write
+  // them with the queue's own (coalesced) order, and the shade kernel reads
+  // them without waiting for the path id.
   RayArrays<T> ray[2];
-  T *bx, *by, *bz;                 // throughput
-  T *lx, *ly, *lz;                 // radiance
-  uint8_t* spec;                   // specular_bounce
+  T *lx, *ly, *lz;                 // radiance (by path id)
   T *ht, *hb1, *hb2;               // hit t, barycentrics (by queue slot)
   uint32_t* hprim;                 // hit primitive (0xFFFFFFFF none; by queue slot)
   int32_t* hplane;                 // hit plane (-1 none; by queue slot)
@@ -361,12 +363,10 @@ __device__ __forceinline__ void generate_body(const DevScene& S, const PathBuffe
   const RayArrays<T>& R = B.ray[0];
   R.ox[pid] = o.x; R.oy[pid] = o.y; R.oz[pid] = o.z;
   R.dx[pid] = d.x; R.dy[pid] = d.y; R.dz[pid] = d.z;
-  B.bx[pid] = 1; B.by[pid] = 1; B.bz[pid] = 1;
   B.lx[pid] = 0; B.ly[pid] = 0; B.lz[pid] = 0;
   if constexpr (sizeof(T) == 4) {
     B.slx[pid] = 0; B.sly[pid] = 0; B.slz[pid] = 0;
   }
-  B.spec[pid] = 1;
 }
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_generate(DevScene S, PathBuffers<T> B, ChunkParams P) {
@@ -436,7 +436,8 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
   bool alive = false, shadow = false;
   uint32_t pid = 0, shadow_cls = 0;
   ShadowReq<T> req;
-  V3<T> nro, ndir;  // the next ray of a path that lives on
+  V3<T> nro, ndir, nbeta;  // the next ray and throughput of a path that lives on
+  bool nspec = false;
   if (i < qv.total) {
     const uint32_t k = active_slot(P, pre, b, i);
     const RayArrays<T>& R = B.ray[b & 1];
@@ -445,7 +446,8 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
     const int plane = B.hplane[k];
     pid = slot_pid(B, b, k);
     const uint32_t pixel = pid_pixel(P, pid), sample = pid_sample(P, pid);
-    V3<T> beta{B.bx[pid], B.by[pid], B.bz[pid]};
+    V3<T> beta{T(1), T(1), T(1)};
+    if (b > 0) beta = V3<T>{R.bx[k], R.by[k], R.bz[k]};
     // The path's radiance is read and written only by a bounce that adds
     // to it (a miss, or emission counted here); most bounces add nothing.
     if (prim == 0xFFFFFFFFu && plane < 0) {
@@ -454,7 +456,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
       B.lx[pid] = L.x; B.ly[pid] = L.y; B.lz[pid] = L.z;
     } else {  // the origin and the specular flag are read by hits only
       const V3<T> o{R.ox[k], R.oy[k], R.oz[k]};
-      const bool spec = B.spec[pid] != 0;
+      const bool spec = b == 0 || R.spec[k] != 0;
       auto rng = DrawSrc<T>::make(B, P, pid, pixel, sample, (uint32_t)b);
       HitRec<T> rec;
       ModeOps<T>::finalize(S, o, d, B.ht[k], B.hb1[k], B.hb2[k], prim, plane, rec);
@@ -492,9 +494,9 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
         if (live && b + 1 < P.max_depth) {
           nro = offset_ray_origin(rec.p, rec.gn, sr.dir);
           ndir = sr.dir;
+          nbeta = beta;
+          nspec = sr.specular;
           alive = true;
-          B.bx[pid] = beta.x; B.by[pid] = beta.y; B.bz[pid] = beta.z;
-          B.spec[pid] = sr.specular ? 1 : 0;
         }
       }
       DrawSrc<T>::store(B, pid, rng);
@@ -511,6 +513,8 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
     B.q[(b + 1) & 1][k] = pid;
     R.ox[k] = nro.x; R.oy[k] = nro.y; R.oz[k] = nro.z;
     R.dx[k] = ndir.x; R.dy[k] = ndir.y; R.dz[k] = ndir.z;
+    R.bx[k] = nbeta.x; R.by[k] = nbeta.y; R.bz[k] = nbeta.z;
+    R.spec[k] = nspec ? 1 : 0;
   }
   if (shadow) {
     const int64_t si = P.q_base + a.b + (int64_t)(b & 1) * B.sq_stride;
