@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define RTG_ABI_VERSION 2
+#define RTG_ABI_VERSION 3
 
 typedef int32_t rtg_status;
 enum {
@@ -174,20 +174,28 @@ int rtg_abi_version(void);
 rtg_status rtg_device_info(rtg_context* ctx, char* name, size_t name_len, int* multiprocessors,
                            uint64_t* hbm_bytes);
 
-/* Uploads a compiled scene: flattens the BVH into 64-byte two-child records
- * (float bounds rounded outward) and the primitives into leaf order.
+/* Uploads a compiled scene: flattens the BVH into 32-byte two-child records
+ * (half-float bounds in the scene's node frame, rounded outward) and the
+ * primitives into leaf order.
  * Replaces the scene argument of Renderer (rendering/renderer.hpp:52). */
 rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* desc, rtg_scene** out);
 /* The FP32 node records rtg_scene_create builds from `desc`, computed on
- * the host (no device is touched): 16 floats per interior node of the
- * reference BVH, in its DFS order — left child box lo.xyz, hi.xyz, right
- * child box lo.xyz, hi.xyz (each rounded outward from the double box, so it
- * contains it), then the left and right child refs and the split axis as
- * u32 bit patterns, and a zero. Writes min(cap, *n) records to `out` (may be
- * NULL) and the record count to *n. The layout replaces BvhNode
- * (include/raytracer/geometry/bvh.hpp:20-26) on the device; this entry point
- * exposes it to tools and tests. */
+ * the host (no device is touched), 16 floats per interior node of the
+ * reference BVH, in its DFS order: left child box lo.xyz, hi.xyz, right
+ * child box lo.xyz, hi.xyz — the record's half floats, widened exactly, in
+ * the node frame of rtg_scene_node_frame (world = origin + v * 2^exp2, each
+ * box rounded outward from the double box, so it contains it) — then the
+ * left and right child refs, the split axis of the reference node (the
+ * device record does not hold it) as u32 bit patterns, and a zero. Writes
+ * min(cap, *n) records to `out` (may be NULL) and the record count to *n.
+ * The layout replaces BvhNode (include/raytracer/geometry/bvh.hpp:20-26) on
+ * the device; this entry point exposes it to tools and tests. */
 rtg_status rtg_scene_node_records(const rtg_scene_desc* desc, float* out, uint64_t cap, uint64_t* n);
+/* The node frame of `desc`'s FP32 node records: the centre of the BVH's
+ * root box (as floats) and the power of two the records' coordinates are
+ * scaled by. No counterpart in the reference (its nodes hold doubles);
+ * for tools and tests that read the records. */
+rtg_status rtg_scene_node_frame(const rtg_scene_desc* desc, float origin[3], int32_t* exp2);
 rtg_status rtg_scene_release(rtg_scene* scene);
 /* A handle on `base`'s device arrays with another camera — the same scene
  * rendered at another image size needs the camera set up for that size

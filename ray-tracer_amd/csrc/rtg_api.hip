@@ -209,6 +209,53 @@ float bits_f(uint32_t u) {
 }
 float4 f4(float x, float y, float z, float w) { return float4{x, y, z, w}; }
 
+struct NodeFrame {
+  float origin[3];
+  int exp2;
+};
+// The frame of the FP32 node records: box coordinates are stored as half
+// floats of (b - origin) * 2^-e, origin the centre of the BVH's root box
+// (a float) and e the power of two that brings the root box within
+// +-2^14 of it (clamped to [-60, 20]; a box past the half range becomes
+// infinite, which still bounds it). A record's precision then follows the
+// scene's own extent, not where the scene sits in world space.
+NodeFrame node_frame(const rtg_scene_desc* d) {
+  NodeFrame f{{0.0f, 0.0f, 0.0f}, 0};
+  if (!d->n_nodes) return f;
+  const rtg_bvh_node& r = d->nodes[0];
+  const double lo[3] = {r.lo.x, r.lo.y, r.lo.z}, hi[3] = {r.hi.x, r.hi.y, r.hi.z};
+  double half = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double c = 0.5 * lo[a] + 0.5 * hi[a];
+    f.origin[a] = std::isfinite(c) ? (float)c : 0.0f;
+    half = std::max({half, std::fabs(lo[a] - f.origin[a]), std::fabs(hi[a] - f.origin[a])});
+  }
+  int e = 0;
+  if (half > 0.0 && std::isfinite(half)) e = (int)std::ceil(std::log2(half / 16384.0));
+  f.exp2 = std::min(20, std::max(-60, e));
+  return f;
+}
+// A coordinate in the frame as the nearest half float at or below (lo) or
+// at or above (hi) it, so the half box contains the double box: v - origin
+// is split exactly into s + err (Knuth's two-sum), both scaled by 2^-e
+// (exact), and the nearest half of s is stepped outward when it lies inside
+// s + err (|err| is below half an ulp of s, far below a half step).
+uint32_t half_bits(_Float16 h) { return __builtin_bit_cast(uint16_t, h); }
+uint32_t frame_half(const NodeFrame& f, int axis, double v, bool up) {
+  const double a = v, b = -(double)f.origin[axis];
+  const double s = a + b, bb = s - a, err = (a - (s - bb)) + (b - bb);
+  const double x = std::ldexp(s, -f.exp2), xe = std::ldexp(err, -f.exp2);
+  const _Float16 h = (_Float16)x;
+  const double hv = (double)h;
+  uint32_t bits = half_bits(h);
+  if (!up && (hv > x || (hv == x && xe < 0.0)))  // the next half down
+    bits = bits == 0u ? 0x8001u : (bits & 0x8000u) ? bits + 1u : bits - 1u;
+  if (up && (hv < x || (hv == x && xe > 0.0)))  // the next half up
+    bits = bits == 0x8000u ? 0x0001u : (bits & 0x8000u) ? bits - 1u : bits + 1u;
+  return bits;
+}
+float half_value(uint32_t b) { return (float)__builtin_bit_cast(_Float16, (uint16_t)b); }
+
 // The camera as generate() reads it: FP64 for the exact mode, FP32 origin,
 // corner - origin, du, dv, u, v and lens radius for the fast mode.
 void set_camera(DevScene& S, const rtg_camera& c) {
@@ -332,8 +379,8 @@ const LaunchHooks* hooks_of(rtg_context* ctx) {
 // rounded outward from the double boxes, so each float box contains its
 // double box, and the two child refs (leaf: first slot + count; interior:
 // record index) and the split axis. max_depth = the BVH's depth.
-bool build_node_records(const rtg_scene_desc* d, std::vector<float4>& node2, uint32_t& max_depth,
-                        std::string& err) {
+bool build_node_records(const rtg_scene_desc* d, const NodeFrame& fr, std::vector<float4>& node2,
+                        uint32_t& max_depth, std::string& err) {
   std::vector<int32_t> n2_index(d->n_nodes, -1);
   uint32_t n_interior = 0;
   max_depth = 0;
@@ -377,10 +424,20 @@ bool build_node_records(const rtg_scene_desc* d, std::vector<float4>& node2, uin
     const rtg_bvh_node& L = d->nodes[i + 1];
     const rtg_bvh_node& R = d->nodes[n.offset];
     float4* o = &node2[(size_t)n2_index[i] * kNode2Stride];
-    o[0] = f4(round_down(L.lo.x), round_down(L.lo.y), round_down(L.lo.z), round_up(L.hi.x));
-    o[1] = f4(round_up(L.hi.y), round_up(L.hi.z), round_down(R.lo.x), round_down(R.lo.y));
-    o[2] = f4(round_down(R.lo.z), round_up(R.hi.x), round_up(R.hi.y), round_up(R.hi.z));
-    o[3] = f4(bits_f(child_ref(i + 1)), bits_f(child_ref(n.offset)), bits_f(n.axis), 0.0f);
+    uint32_t h[12];
+    const rtg_bvh_node* child[2] = {&L, &R};
+    for (int c = 0; c < 2; ++c) {
+      const double lo[3] = {child[c]->lo.x, child[c]->lo.y, child[c]->lo.z};
+      const double hi[3] = {child[c]->hi.x, child[c]->hi.y, child[c]->hi.z};
+      for (int a = 0; a < 3; ++a) {
+        h[6 * c + a] = frame_half(fr, a, lo[a], false);
+        h[6 * c + 3 + a] = frame_half(fr, a, hi[a], true);
+      }
+    }
+    o[0] = f4(bits_f(h[0] | h[1] << 16), bits_f(h[2] | h[3] << 16), bits_f(h[4] | h[5] << 16),
+              bits_f(h[6] | h[7] << 16));
+    o[1] = f4(bits_f(h[8] | h[9] << 16), bits_f(h[10] | h[11] << 16), bits_f(child_ref(i + 1)),
+              bits_f(child_ref(n.offset)));
   }
   return true;
 }
@@ -520,9 +577,34 @@ rtg_status rtg_scene_node_records(const rtg_scene_desc* desc, float* out, uint64
   std::vector<float4> node2;
   uint32_t depth = 0;
   std::string err;
-  if (!build_node_records(desc, node2, depth, err)) return fail(nullptr, RTG_ERR_UNSUPPORTED, err);
+  if (!build_node_records(desc, node_frame(desc), node2, depth, err)) return fail(nullptr, RTG_ERR_UNSUPPORTED, err);
   *n = node2.size() / kNode2Stride;
-  if (out) std::memcpy(out, node2.data(), std::min<uint64_t>(cap, *n) * kNode2Stride * sizeof(float4));
+  const uint64_t m = std::min<uint64_t>(cap, *n);
+  for (uint64_t r = 0; out && r < m; ++r) {  // 16 floats per record: the 12 half planes widened, refs, axis, 0
+    uint32_t w[8];
+    std::memcpy(w, &node2[r * kNode2Stride], sizeof w);
+    float* o = out + 16 * r;
+    for (int k = 0; k < 12; ++k) o[k] = half_value((w[k / 2] >> (16 * (k & 1))) & 0xFFFFu);
+    o[12] = bits_f(w[6]);
+    o[13] = bits_f(w[7]);
+    o[14] = 0.0f;
+    o[15] = 0.0f;
+  }
+  // the split axis of the reference node each record stands for
+  if (out && m) {
+    uint64_t r = 0;
+    for (uint32_t i = 0; i < desc->n_nodes && r < m; ++i)
+      if (desc->nodes[i].count == 0) out[16 * r++ + 14] = bits_f(desc->nodes[i].axis);
+  }
+  return RTG_OK;
+}
+
+rtg_status rtg_scene_node_frame(const rtg_scene_desc* desc, float origin[3], int32_t* exp2) {
+  if (!desc || !origin || !exp2) return fail(nullptr, RTG_ERR_INVALID_ARG, "rtg_scene_node_frame: NULL argument");
+  if (desc->n_nodes && !desc->nodes) return fail(nullptr, RTG_ERR_INVALID_ARG, "scene: null array with non-zero count");
+  const NodeFrame f = node_frame(desc);
+  for (int a = 0; a < 3; ++a) origin[a] = f.origin[a];
+  *exp2 = f.exp2;
   return RTG_OK;
 }
 
@@ -546,9 +628,10 @@ rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* d, rtg_scene
   // --- BVH depth and the FP32 two-child node layout ---
   std::vector<float4> node2;
   uint32_t max_depth = 0;
+  const NodeFrame frame = node_frame(d);
   {
     std::string err;
-    if (!build_node_records(d, node2, max_depth, err)) return cleanup(fail(ctx, RTG_ERR_UNSUPPORTED, err));
+    if (!build_node_records(d, frame, node2, max_depth, err)) return cleanup(fail(ctx, RTG_ERR_UNSUPPORTED, err));
   }
   if (np > kMaxSlots) return cleanup(fail(ctx, RTG_ERR_UNSUPPORTED, "scene: more than 2^25 primitives"));
   s->stack = max_depth + 1;
@@ -679,9 +762,14 @@ rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* d, rtg_scene
   if (d->n_nodes) {
     const rtg_bvh_node& r = d->nodes[0];
     S.root_ref = r.count > 0 ? leaf_ref(r.offset, r.count) : 0u;
-    S.root_lo[0] = round_down(r.lo.x); S.root_lo[1] = round_down(r.lo.y); S.root_lo[2] = round_down(r.lo.z);
-    S.root_hi[0] = round_up(r.hi.x); S.root_hi[1] = round_up(r.hi.y); S.root_hi[2] = round_up(r.hi.z);
+    const double lo[3] = {r.lo.x, r.lo.y, r.lo.z}, hi[3] = {r.hi.x, r.hi.y, r.hi.z};
+    for (int a = 0; a < 3; ++a) {  // in the node frame, as the records' boxes
+      S.root_lo[a] = half_value(frame_half(frame, a, lo[a], false));
+      S.root_hi[a] = half_value(frame_half(frame, a, hi[a], true));
+    }
   }
+  for (int a = 0; a < 3; ++a) S.frame_o[a] = frame.origin[a];
+  S.frame_s = std::ldexp(1.0f, frame.exp2);
   set_camera(S, d->camera);
   S.bg = d->background;
   const rtg_background& b = d->background;

@@ -379,7 +379,7 @@ __global__ __launch_bounds__(kBlock) void k_dfirst(DevScene S, ChunkParams P, Di
       camera32(S, (int)(pixel % P.width), (int)(pixel / P.width), jx, jy, rng, o, d);
     }
     Counters c{0, 0};
-    const Hit32 h = closest32(S, P.use_bvh != 0, make_ray32(o.x, o.y, o.z, d.x, d.y, d.z), 1e-6f, __builtin_inff(),
+    const Hit32 h = closest32(S, P.use_bvh != 0, make_ray32(S, o.x, o.y, o.z, d.x, d.y, d.z), 1e-6f, __builtin_inff(),
                               lds_stack(sref), c);
     st[kStRays] = 1;
     st[kStVisits] = c.visits;
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(kBlock) void k_dfirst(DevScene S, ChunkParams P, Di
         ShadowReq<float> req;
         if (!direct_light(S, key, 0u, li, rec, mat, w, req)) continue;
         Counters c{0, 0};
-        if (!any32(S, P.use_bvh != 0, make_ray32(req.o.x, req.o.y, req.o.z, req.d.x, req.d.y, req.d.z), 1e-6f,
+        if (!any32(S, P.use_bvh != 0, make_ray32(S, req.o.x, req.o.y, req.o.z, req.d.x, req.d.y, req.d.z), 1e-6f,
                    req.tmax, lds_stack(sref), c))
           f.add(req.f_rop);
         ++st[kStShadow];
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void k_dextend(DevScene S, ChunkParams P, D
     const uint32_t k = queue_slot(pre, M.sub_gen, i);
     const float4 a = M.e0[g & 1][k], b = M.e1[g & 1][k];
     Counters c{0, 0};
-    const Hit32 h = closest32(S, P.use_bvh != 0, make_ray32(a.x, a.y, a.z, b.x, b.y, b.z), 1e-6f, __builtin_inff(),
+    const Hit32 h = closest32(S, P.use_bvh != 0, make_ray32(S, a.x, a.y, a.z, b.x, b.y, b.z), 1e-6f, __builtin_inff(),
                               lds_stack(sref), c);
     M.hit[g & 1][k] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
     M.plane[g & 1][k] = h.plane;
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_dshadow(DevScene S, ChunkParams P, D
       if (b.w == 0.0f) continue;
       const float4 a = M.s0[r];
       Counters c{0, 0};
-      if (!any32(S, P.use_bvh != 0, make_ray32(a.x, a.y, a.z, b.x, b.y, b.z), 1e-6f, a.w, lds_stack(sref), c))
+      if (!any32(S, P.use_bvh != 0, make_ray32(S, a.x, a.y, a.z, b.x, b.y, b.z), 1e-6f, a.w, lds_stack(sref), c))
         vis |= 1u << li;
       ++st[kStShadow];
       st[kStSVisits] += c.visits;
@@ -605,7 +605,7 @@ __global__ __launch_bounds__(kBlock) void k_ddfs(DevScene S, ChunkParams P, Dire
     while (true) {
       ++st[kStRays];
       Counters c{0, 0};
-      const Hit32 h = closest32(S, P.use_bvh != 0, make_ray32(o.x, o.y, o.z, d.x, d.y, d.z), 1e-6f,
+      const Hit32 h = closest32(S, P.use_bvh != 0, make_ray32(S, o.x, o.y, o.z, d.x, d.y, d.z), 1e-6f,
                                 __builtin_inff(), lds_stack(sref), c);
       st[kStVisits] += c.visits;
       st[kStTests] += c.tests;
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(kBlock) void k_ddfs(DevScene S, ChunkParams P, Dire
           if (!direct_light(S, key, node, li, rec, mat, w, req)) continue;
           ++st[kStShadow];
           Counters c2{0, 0};
-          if (!any32(S, P.use_bvh != 0, make_ray32(req.o.x, req.o.y, req.o.z, req.d.x, req.d.y, req.d.z), 1e-6f,
+          if (!any32(S, P.use_bvh != 0, make_ray32(S, req.o.x, req.o.y, req.o.z, req.d.x, req.d.y, req.d.z), 1e-6f,
                      req.tmax, lds_stack(sref), c2))
             f.add(req.f_rop);
           st[kStSVisits] += c2.visits;

@@ -253,12 +253,12 @@ template <>
 struct ModeOps<float> {
   using R = HashDraws;
   __device__ static Hit32 closest(const DevScene& S, bool bvh, V3<float> o, V3<float> d, LdsStack stk, Counters& c) {
-    const Ray32 r = make_ray32(o.x, o.y, o.z, d.x, d.y, d.z);
+    const Ray32 r = make_ray32(S, o.x, o.y, o.z, d.x, d.y, d.z);
     return closest32(S, bvh, r, 1e-6f, __builtin_inff(), stk, c);
   }
   __device__ static bool occluded(const DevScene& S, bool bvh, V3<float> o, V3<float> d, float tmax, LdsStack stk,
                                   Counters& c) {
-    const Ray32 r = make_ray32(o.x, o.y, o.z, d.x, d.y, d.z);
+    const Ray32 r = make_ray32(S, o.x, o.y, o.z, d.x, d.y, d.z);
     return any32(S, bvh, r, 1e-6f, tmax, stk, c);
   }
   __device__ static void finalize(const DevScene& S, V3<float> o, V3<float> d, float t, float b1, float b2,
@@ -629,7 +629,7 @@ __device__ __forceinline__ void extend_range(const DevScene& S, const PathBuffer
     if (refill(cursor, rg, __ballot(!active), more, i) && !active && i < rg.r1) {
       k = active_slot(P, pre, b, i);
       const RayArrays<float>& R = B.ray[b & 1];
-      r = make_ray32(R.ox[k], R.oy[k], R.oz[k], R.dx[k], R.dy[k], R.dz[k]);
+      r = make_ray32(S, R.ox[k], R.oy[k], R.oz[k], R.dx[k], R.dy[k], R.dz[k]);
       ++st[kStRays];
       if (P.use_bvh) {
         active = closest_begin32(S, r, 1e-6f, __builtin_inff(), stk, c, T);
@@ -662,7 +662,7 @@ __device__ __forceinline__ void shadow_range(const DevScene& S, const PathBuffer
   Counters c{0, 0};
   Walk w{kNoRef, kNoRef, 0, kNoRef};
   Ray32 r{};
-  uint32_t k = 0, neg = 0;
+  uint32_t k = 0;
   float tmax = 0.0f;
   bool active = false, more = true, hit = false;
   auto finish = [&]() {
@@ -677,10 +677,10 @@ __device__ __forceinline__ void shadow_range(const DevScene& S, const PathBuffer
     uint32_t i = 0;
     if (refill(cursor, rg, __ballot(!active), more, i) && !active && i < rg.r1) {
       k = P.q_base + queue_slot(pre, P.sub_cap, i) + (uint32_t)(b & 1) * B.sq_stride;
-      r = make_ray32(B.sox[k], B.soy[k], B.soz[k], B.sdx[k], B.sdy[k], B.sdz[k]);
+      r = make_ray32(S, B.sox[k], B.soy[k], B.soz[k], B.sdx[k], B.sdy[k], B.sdz[k]);
       tmax = B.stm[k];
       if (P.use_bvh) {
-        active = any_begin32(S, r, 1e-6f, tmax, stk, c, w, neg, hit);
+        active = any_begin32(S, r, 1e-6f, tmax, stk, c, w, hit);
       } else {
         hit = any32(S, false, r, 1e-6f, tmax, stk, c);
         active = false;
@@ -691,7 +691,7 @@ __device__ __forceinline__ void shadow_range(const DevScene& S, const PathBuffer
       if (!more) break;
       continue;
     }
-    if (active && any_round32(S, r, 1e-6f, tmax, stk, c, w, neg, hit)) {
+    if (active && any_round32(S, r, 1e-6f, tmax, stk, c, w, hit)) {
       finish();
       active = false;
     }
@@ -1045,7 +1045,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const double* __re
         out.flags = any64(S, use_bvh != 0, o, d, 1e-6, r[7], stk, c) ? 1u : 0u;
       }
     } else {
-      const Ray32 ray = make_ray32((float)r[0], (float)r[1], (float)r[2], (float)r[3], (float)r[4], (float)r[5]);
+      const Ray32 ray = make_ray32(S, (float)r[0], (float)r[1], (float)r[2], (float)r[3], (float)r[4], (float)r[5]);
       if (mode == RTG_TRACE_CLOSEST) {
         const Hit32 h = closest32(S, use_bvh != 0, ray, (float)r[6], (float)r[7], stk, c);
         if (h.prim != 0xFFFFFFFFu || h.plane >= 0) {

@@ -2,11 +2,14 @@
 // code (rtg_api.cpp) and the kernels.
 //
 // FP32 traversal data (the fast path):
-//   node2  — one 64-byte record per INTERIOR node of the reference BVH, in
-//            the reference's DFS order: the float boxes of both children
-//            (rounded outward from the double boxes) and two child refs.
+//   node2  — one 32-byte record per INTERIOR node of the reference BVH, in
+//            the reference's DFS order: the boxes of both children as half
+//            floats in the scene's node frame ((b - frame_o) / frame_s,
+//            rounded outward from the double boxes) and two child refs.
 //            Testing both children from one record halves the dependent
-//            loads per level compared with the reference's one-box nodes.
+//            loads per level compared with the reference's one-box nodes;
+//            half floats halve the bytes each level moves through the
+//            vector memory pipeline, which bounds the traversal kernels.
 //   slot32 — primitives in leaf order (slot k = prim_indices[k]), 48 bytes:
 //            triangle {v0, id} {e1} {e2}; sphere {centre, id} {r, r^2}.
 // FP64 traversal data (the exact mode): the reference's own node and
@@ -48,7 +51,7 @@ enum : uint32_t { kL32Point = 0, kL32Sphere = 1, kL32Triangle = 2 };
 constexpr int kLight32Stride = 5;  // float4 per light
 constexpr int kShade32Stride = 4;  // float4 per primitive
 constexpr int kSlot32Stride = 3;   // float4 per slot
-constexpr int kNode2Stride = 4;    // float4 per interior node
+constexpr int kNode2Stride = 2;    // float4 per interior node
 
 struct DevScene {
   // FP32 traversal
@@ -56,7 +59,8 @@ struct DevScene {
   const float4* slot32;
   uint32_t root_ref;  // child ref of the root (interior: 0)
   uint32_t has_bvh;
-  float root_lo[3], root_hi[3];
+  float root_lo[3], root_hi[3];  // the root box in the node frame (half values)
+  float frame_o[3], frame_s;     // node frame: world = frame_o + frame_s * frame coordinate
   uint32_t stack_depth;  // BVH depth + 1
   uint32_t n_slots;
   // FP64 (reference layout)

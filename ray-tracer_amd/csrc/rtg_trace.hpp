@@ -293,8 +293,13 @@ __device__ inline bool any64(const DevScene& S, bool use_bvh, V3<double> o, V3<d
 // plane is entered first): every entry moves down and every exit up by at
 // least 4x the absolute error, at no cost per box. The relative part is
 // covered by widening the exit distance (box32).
+// A ray for the FP32 walks: origin and direction (primitive tests), and
+// its slab terms in the scene's node frame (node boxes are half floats of
+// (b - frame_o) / frame_s): a box plane h lies at t = fma(h, i, -q) with
+// i = frame_s / d and q = (o - frame_o) / d, q moved by a margin per side.
 struct Ray32 {
-  float ox, oy, oz, dx, dy, dz, ix, iy, iz;
+  float ox, oy, oz, dx, dy, dz;
+  float ix, iy, iz;     // frame_s / d
   float lqx, lqy, lqz;  // offsets of the lo planes
   float hqx, hqy, hqz;  // offsets of the hi planes
 };
@@ -306,24 +311,28 @@ __device__ __forceinline__ float safe_inv(float d) {
   return 1.0f / a;
 }
 
-__device__ __forceinline__ Ray32 make_ray32(float ox, float oy, float oz, float dx, float dy, float dz) {
+// The margin covers the rounding of q (the origin's offset from the frame
+// and the product, each within half an ulp of q) with room to spare (8 ulp).
+__device__ __forceinline__ Ray32 make_ray32(const DevScene& S, float ox, float oy, float oz, float dx, float dy,
+                                            float dz) {
   Ray32 r;
   r.ox = ox; r.oy = oy; r.oz = oz;
   r.dx = dx; r.dy = dy; r.dz = dz;
-  r.ix = safe_inv(dx); r.iy = safe_inv(dy); r.iz = safe_inv(dz);
-  const float qx = ox * r.ix, qy = oy * r.iy, qz = oz * r.iz;
+  const float ix = safe_inv(dx), iy = safe_inv(dy), iz = safe_inv(dz);
+  r.ix = ix * S.frame_s; r.iy = iy * S.frame_s; r.iz = iz * S.frame_s;
+  const float qx = (ox - S.frame_o[0]) * ix, qy = (oy - S.frame_o[1]) * iy, qz = (oz - S.frame_o[2]) * iz;
   constexpr float k = 0x1p-21f;
-  const float mx = r_copysign(r_fabs(qx) * k, r.ix), my = r_copysign(r_fabs(qy) * k, r.iy),
-              mz = r_copysign(r_fabs(qz) * k, r.iz);
+  const float mx = r_copysign(r_fabs(qx) * k, ix), my = r_copysign(r_fabs(qy) * k, iy),
+              mz = r_copysign(r_fabs(qz) * k, iz);
   r.lqx = qx + mx; r.lqy = qy + my; r.lqz = qz + mz;
   r.hqx = qx - mx; r.hqy = qy - my; r.hqz = qz - mz;
   return r;
 }
 
-// Slab test of one float box; returns the entry distance in tn. Plane
-// distances are fma(bound, 1/d, -offset) with the ray's margins folded into
-// the offsets (Ray32); the far distance is also widened by 5e-7 relative, so
-// rounding never culls a box the ray touches.
+// Slab test of one box in the node frame; returns the entry distance in tn.
+// Plane distances are fma(bound, frame_s/d, -offset) with the ray's margins
+// folded into the offsets (Ray32); the far distance is also widened by 5e-7
+// relative, so rounding never culls a box the ray touches.
 __device__ __forceinline__ bool box32(const Ray32& r, float lx, float ly, float lz, float hx, float hy, float hz,
                                       float tmin, float tmax, float& tn) {
   const float t0x = __builtin_fmaf(lx, r.ix, -r.lqx), t1x = __builtin_fmaf(hx, r.ix, -r.hqx);
@@ -353,20 +362,6 @@ __device__ __forceinline__ float4 ld_global(const float4* p) {
 __device__ __forceinline__ uint4 ld_global(const uint4* p) {
   const NativeU4 v = *(const __attribute__((address_space(1))) NativeU4*)p;
   return uint4{v.x, v.y, v.z, v.w};
-}
-// The child refs and split axis of a node record (its fourth float4; the
-// last word is padding). The load still writes the padding word's register
-// when it returns; the compiler, seeing that word unused, may hand its
-// register to another value meanwhile and must then wait for the load before
-// writing it (measured: the traversal stack's LDS accesses waited for the
-// node's first load). The empty asm statement keeps the padding word live
-// until the node's boxes are tested.
-struct NodeRefs {
-  uint32_t x, y, z, pad;
-};
-__device__ __forceinline__ NodeRefs ld_node_refs(const float4* p) {
-  const uint4 v = ld_global(reinterpret_cast<const uint4*>(p));
-  return NodeRefs{v.x, v.y, v.z, v.w};
 }
 __device__ __forceinline__ uint32_t ld_global(const uint32_t* p) {
   return *(const __attribute__((address_space(1))) uint32_t*)p;
@@ -464,13 +459,35 @@ __device__ __forceinline__ int planes32(const DevScene& S, const Ray32& r, float
   return hit;
 }
 
-// Inner node record `ref` (< 2^25 records of 64 bytes): a 32-bit byte
+// Inner node record `ref` (< 2^25 records of 32 bytes): a 32-bit byte
 // offset from the array base.
 __device__ __forceinline__ const float4* node_at(const float4* __restrict__ node2, uint32_t ref) {
-  return reinterpret_cast<const float4*>(reinterpret_cast<const char*>(node2) +
-                                         (ref << 6));
+  return reinterpret_cast<const float4*>(reinterpret_cast<const char*>(node2) + (ref << 5));
 }
-static_assert(kNode2Stride * sizeof(float4) == 64, "two-child node records are 64 bytes");
+static_assert(kNode2Stride * sizeof(float4) == 32, "two-child node records are 32 bytes");
+// The half floats of a record word (low and high half).
+__device__ __forceinline__ float half_lo(uint32_t w) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xFFFFu));
+}
+__device__ __forceinline__ float half_hi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
+// Both children's boxes of a record ({L lo, L hi, R lo, R hi} as 12 halves,
+// then the left and right child refs); returns the refs.
+struct NodeTest {
+  uint32_t l, r;
+  bool hl, hr;
+  float tl, tr;
+};
+__device__ __forceinline__ NodeTest node_test(const float4* n, const Ray32& ray, float tmin, float tmax) {
+  const uint4 a = ld_global(reinterpret_cast<const uint4*>(n)), b = ld_global(reinterpret_cast<const uint4*>(n + 1));
+  NodeTest t;
+  t.hl = box32(ray, half_lo(a.x), half_hi(a.x), half_lo(a.y), half_hi(a.y), half_lo(a.z), half_hi(a.z), tmin, tmax,
+               t.tl);
+  t.hr = box32(ray, half_lo(a.w), half_hi(a.w), half_lo(b.x), half_hi(b.x), half_lo(b.y), half_hi(b.y), tmin, tmax,
+               t.tr);
+  t.l = b.z;
+  t.r = b.w;
+  return t;
+}
 
 struct Hit32 {
   float t, b1, b2;
@@ -561,7 +578,6 @@ struct Walk {
 struct Trav32 {
   Hit32 h;
   Walk w;
-  uint32_t neg;  // direction signs (bit per axis)
 };
 // Closest hit: planes, then the root box; false when that already ends the
 // walk (h final).
@@ -572,7 +588,6 @@ __device__ __forceinline__ bool closest_begin32(const DevScene& S, const Ray32& 
   T.h.prim = 0xFFFFFFFFu;
   T.h.b1 = T.h.b2 = 0.0f;
   T.w = Walk{kNoRef, kNoRef, 0, kNoRef};
-  T.neg = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
   if (!S.has_bvh) return false;
   float tn;
   ++c.visits;
@@ -590,15 +605,10 @@ __device__ __forceinline__ bool closest_round32(const DevScene& S, const Ray32& 
   Hit32& h = T.h;
   for (uint32_t steps = 1; w.inner(); ++steps) {
     const uint32_t under = w.below(stk);
-    const float4* n = node_at(S.node2, w.ref);
-    const float4 a = ld_global(n), b = ld_global(n + 1), cc = ld_global(n + 2);
-    const NodeRefs m = ld_node_refs(n + 3);
-    float tl, tr;
-    const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, h.t, tl);
-    const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, h.t, tr);
+    const NodeTest nt = node_test(node_at(S.node2, w.ref), r, tmin, h.t);
     c.visits += 2;
-    asm volatile("" ::"v"(m.pad));
-    w.step(stk, under, m.x, m.y, hl, hr, (T.neg >> m.z) & 1u);
+    // the nearer child first (by entry distance)
+    w.step(stk, under, nt.l, nt.r, nt.hl, nt.hr, nt.tr < nt.tl);
     if (w.descended(steps)) break;
   }
   w.park(stk);  // a leaf popped after a miss
@@ -653,9 +663,8 @@ __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& 
 // Any hit in (tmin, tmax): planes, then the root box; false when that
 // already ends the walk (`hit` final).
 __device__ __forceinline__ bool any_begin32(const DevScene& S, const Ray32& r, float tmin, float tmax, LdsStack stk,
-                                            Counters& c, Walk& w, uint32_t& neg, bool& hit) {
+                                            Counters& c, Walk& w, bool& hit) {
   w = Walk{kNoRef, kNoRef, 0, kNoRef};
-  neg = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
   hit = false;
   for (uint32_t i = 0; i < S.n_planes; ++i) {
     const float4 p = ld_global(S.plane32 + 2 * i), n = ld_global(S.plane32 + 2 * i + 1);
@@ -678,22 +687,15 @@ __device__ __forceinline__ bool any_begin32(const DevScene& S, const Ray32& r, f
 }
 // One round of the any-hit walk; true when it has ended (`hit` set on a hit).
 __device__ __forceinline__ bool any_round32(const DevScene& S, const Ray32& r, float tmin, float tmax, LdsStack stk,
-                                            Counters& c, Walk& w, uint32_t neg, bool& hit) {
+                                            Counters& c, Walk& w, bool& hit) {
   for (uint32_t steps = 1; w.inner(); ++steps) {
     const uint32_t under = w.below(stk);
-    const float4* n = node_at(S.node2, w.ref);
-    const float4 a = ld_global(n), b = ld_global(n + 1), cc = ld_global(n + 2);
-    const NodeRefs m = ld_node_refs(n + 3);
-    float tl, tr;
-    const bool hl = box32(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, tmax, tl);
-    const bool hr = box32(r, b.z, b.w, cc.x, cc.y, cc.z, cc.w, tmin, tmax, tr);
+    const NodeTest nt = node_test(node_at(S.node2, w.ref), r, tmin, tmax);
     c.visits += 2;
     // Any order finds whether some hit lies in (tmin, tmax). Bvh::any
-    // visits the left child first (bvh.hpp:106-132); the far child first
-    // (by the direction's sign on the split axis) is 1.6 % faster here,
-    // near-first 1.4 % slower (torus workload, any-hit kernel alone).
-    asm volatile("" ::"v"(m.pad));
-    w.step(stk, under, m.x, m.y, hl, hr, !((neg >> m.z) & 1u));
+    // visits the left child first (bvh.hpp:106-132); the farther child
+    // first (by entry distance) is faster here.
+    w.step(stk, under, nt.l, nt.r, nt.hl, nt.hr, nt.tr > nt.tl);
     if (w.descended(steps)) break;
   }
   w.park(stk);  // a leaf popped after a miss
@@ -735,10 +737,9 @@ __device__ inline bool any32(const DevScene& S, bool use_bvh, const Ray32& r, fl
     return false;
   }
   Walk w;
-  uint32_t neg;
   bool hit;
-  if (any_begin32(S, r, tmin, tmax, stk, c, w, neg, hit))
-    while (!any_round32(S, r, tmin, tmax, stk, c, w, neg, hit)) {
+  if (any_begin32(S, r, tmin, tmax, stk, c, w, hit))
+    while (!any_round32(S, r, tmin, tmax, stk, c, w, hit)) {
     }
   return hit;
 }

@@ -149,6 +149,7 @@ def load_rtg():
         "rtg_device_info": (C.c_int32, [_VP, C.c_char_p, C.c_size_t, P(C.c_int), P(C.c_uint64)]),
         "rtg_scene_create": (C.c_int32, [_VP, P(SceneDesc), P(_VP)]),
         "rtg_scene_node_records": (C.c_int32, [P(SceneDesc), _VP, C.c_uint64, P(C.c_uint64)]),
+        "rtg_scene_node_frame": (C.c_int32, [P(SceneDesc), P(C.c_float), P(C.c_int32)]),
         "rtg_scene_release": (C.c_int32, [_VP]),
         "rtg_scene_with_camera": (C.c_int32, [_VP, _VP, P(Camera), P(_VP)]),
         "rtg_scene_bytes": (C.c_uint64, [_VP]),
@@ -187,7 +188,7 @@ def load_rtg():
 def rtg_symbols():
     """Names declared in include/rtg.h (checked by the CPU tests)."""
     return ["rtg_open", "rtg_close", "rtg_last_error", "rtg_abi_version", "rtg_device_info", "rtg_scene_create",
-            "rtg_scene_node_records",
+            "rtg_scene_node_records", "rtg_scene_node_frame",
             "rtg_scene_release", "rtg_scene_with_camera", "rtg_scene_bytes", "rtg_context_bytes", "rtg_accum_create", "rtg_accum_release", "rtg_accum_clear",
             "rtg_accum_read", "rtg_accum_min_samples", "rtg_render_pass", "rtg_render_passes", "rtg_render_batch",
             "rtg_render_pass_static", "rtg_resolve",
@@ -213,6 +214,18 @@ def node_records(desc: "SceneDesc"):
     if n.value:
         lib.rtg_scene_node_records(C.byref(desc), _ptr(out), n.value, C.byref(n))
     return out
+
+
+def node_frame(desc: "SceneDesc"):
+    """(origin float32[3], exp2) of the node records' frame: world
+    coordinate = origin + record value * 2**exp2 (rtg_scene_node_frame)."""
+    lib = load_rtg()
+    o = (C.c_float * 3)()
+    e = C.c_int32(0)
+    st = lib.rtg_scene_node_frame(C.byref(desc), o, C.byref(e))
+    if st != 0:
+        raise RtgError(f"rtg status {st}: {lib.rtg_last_error(None).decode()}")
+    return np.array(list(o), np.float32), e.value
 
 
 class Context:

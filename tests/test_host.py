@@ -137,10 +137,11 @@ NODE_SCENES = [("cornell_c1.json", None), ("torus_c2.json", None), ("sphere_fiel
 
 @pytest.mark.parametrize("name,seed", NODE_SCENES)
 def test_fp32_node_boxes_contain_the_double_boxes(name, seed):
-    """Every FP32 child box of the device's node records contains the double
-    box of the reference BVH node it stands for (include/raytracer/geometry/
-    bvh.hpp:20-26), child refs point at the right records or leaf ranges, and
-    the split axis is the node's."""
+    """Every child box of the device's FP32 node records (half floats in the
+    scene's node frame) contains the double box of the reference BVH node it
+    stands for (include/raytracer/geometry/bvh.hpp:20-26) and is the tightest
+    half box that does, child refs point at the right records or leaf ranges,
+    and the reported split axis is the node's."""
     j = scenes.load(name) if name else scenes.random_scene_json(300, 600, seed, seed == 2)
     hs = rtg.HostScene(json.dumps(j))
     d = hs.desc
@@ -148,17 +149,20 @@ def test_fp32_node_boxes_contain_the_double_boxes(name, seed):
                           dtype=np.dtype([("lo", "<f8", 3), ("hi", "<f8", 3), ("offset", "<u4"), ("count", "<u2"),
                                           ("axis", "u1"), ("pad", "u1")]))
     rec = rtg.node_records(d)
+    origin, e = rtg.node_frame(d)
     interior = np.flatnonzero(nodes["count"] == 0)
     assert len(rec) == len(interior) > 0
     record_of = np.full(len(nodes), -1, np.int64)
     record_of[interior] = np.arange(len(interior))
     left, right = interior + 1, nodes["offset"][interior].astype(np.int64)
-    f = rec[:, :12].astype(np.float64)
-    for child, lo, hi in ((left, f[:, 0:3], f[:, 3:6]), (right, f[:, 6:9], f[:, 9:12])):
-        assert (lo <= nodes["lo"][child]).all() and (hi >= nodes["hi"][child]).all()
-        # outward rounding is tight: one float step at most
-        assert (np.nextafter(lo.astype(np.float32), np.float32(np.inf)).astype(np.float64) > nodes["lo"][child]).all()
-        assert (np.nextafter(hi.astype(np.float32), np.float32(-np.inf)).astype(np.float64) < nodes["hi"][child]).all()
+    half = rec[:, :12].astype(np.float16)
+    assert np.array_equal(half.astype(np.float32), rec[:, :12])  # the planes are half floats
+    world = lambda h: origin.astype(np.float64) + h.astype(np.float64) * 2.0 ** e  # exact in double
+    for child, lo, hi in ((left, half[:, 0:3], half[:, 3:6]), (right, half[:, 6:9], half[:, 9:12])):
+        assert (world(lo) <= nodes["lo"][child]).all() and (world(hi) >= nodes["hi"][child]).all()
+        # outward rounding is tight: the next half inward no longer contains the box
+        assert (world(np.nextafter(lo, np.float16(np.inf))) > nodes["lo"][child]).all()
+        assert (world(np.nextafter(hi, np.float16(-np.inf))) < nodes["hi"][child]).all()
     refs = rec[:, 12:15].view(np.uint32)
     assert np.array_equal(refs[:, 2], nodes["axis"][interior])
     for col, child in ((0, left), (1, right)):
