@@ -29,9 +29,12 @@ WORKLOADS = {
 }
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # Algorithmic bytes of the FP32 closest-hit kernel: 32 B per node box tested
-# (a 64-byte record holds both children), 48 B per triangle test, 28 B of ray
-# read (queue slot + origin + direction) and 20 B of hit written per ray.
-NODE_B, TRI_B, RAY_IN_B, HIT_OUT_B, RADIANCE_OUT_B = 32, 48, 28, 20, 12
+# (the per-node figure the measurement is defined with), 48 B per triangle
+# test, 24 B of ray read (origin + direction) and 20 B of hit written per
+# ray. The device's node records are 32 bytes for both children (half-float
+# boxes), so a box tested moves 16 B: reported beside as the layout's bytes.
+NODE_B, TRI_B, RAY_IN_B, HIT_OUT_B, RADIANCE_OUT_B = 32, 48, 24, 20, 12
+NODE_LAYOUT_B = 16
 
 
 def load_pkg():
@@ -142,8 +145,11 @@ def extend_roofline(prof, traffic, kind="extend"):
         bytes_total = (NODE_B * prof["closest_visits"] + TRI_B * prof["closest_tests"] +
                        (RAY_IN_B + HIT_OUT_B) * prof["closest_rays"])
     per_launch = bytes_total / n_launch
+    visits = prof["closest_visits"] + (prof["shadow_visits"] if kind == "direct" else 0)
+    layout_per_launch = (bytes_total - (NODE_B - NODE_LAYOUT_B) * visits) / n_launch
     avg_ms = ext["ms"] / n_launch
     achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    layout_achieved = layout_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     # What the HBM itself carries per launch (FETCH_SIZE + WRITE_SIZE counters,
     # scaled by a copy of known size) over the same launch time.
     hbm = traffic / (avg_ms * 1e-3) / 1e9 if traffic and avg_ms > 0 else None
@@ -153,6 +159,8 @@ def extend_roofline(prof, traffic, kind="extend"):
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "hbm_achieved": hbm, "hbm_frac": hbm / HBM_PEAK_GBS if hbm is not None else None,
             "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_ms, "launches": ext["launches"],
+            "layout_bytes_per_launch": layout_per_launch, "layout_achieved": layout_achieved,
+            "layout_frac": layout_achieved / HBM_PEAK_GBS,
             "node_visits_per_ray": prof["closest_visits"] / max(1, prof["closest_rays"]),
             "tests_per_ray": prof["closest_tests"] / max(1, prof["closest_rays"]),
             "shadow_node_visits_per_ray": prof["shadow_visits"] / max(1, prof["shadow_rays"]),
@@ -533,7 +541,9 @@ def main():
         # the closest-hit and any-hit kernels together) over its wall time:
         # the node-traffic rate the GPU sustains across concurrent kernels.
         "node_traffic": {"bytes_per_step": node_bytes // args.steps, "achieved": node_rate, "unit": "GB/s",
-                         "peak": HBM_PEAK_GBS, "frac": node_rate / HBM_PEAK_GBS},
+                         "peak": HBM_PEAK_GBS, "frac": node_rate / HBM_PEAK_GBS,
+                         "bytes_per_box": NODE_B, "layout_bytes_per_box": NODE_LAYOUT_B,
+                         "layout_frac": node_rate * NODE_LAYOUT_B / NODE_B / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "f64": f64,
         "batched": batch,
