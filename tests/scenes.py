@@ -84,3 +84,25 @@ def random_scene_json(spheres, triangles, seed, with_mesh):
         elif o.kind == 1:
             objs.append({"type": "plane", "point": list(o.a), "normal": list(o.b), "material": "m"})
     return {"materials": {"m": {"type": "lambertian"}}, "objects": objs}
+
+
+def translated(j, off):
+    """Scene JSON `j` (spheres, triangles, meshes, planes) moved by `off`."""
+    add = lambda v: [v[0] + off[0], v[1] + off[1], v[2] + off[2]]
+    out = dict(j)
+    objs = []
+    for o in j["objects"]:
+        o = dict(o)
+        if o["type"] == "sphere":
+            o["center"] = add(o["center"])
+        elif o["type"] == "triangle":
+            o["v0"], o["v1"], o["v2"] = add(o["v0"]), add(o["v1"]), add(o["v2"])
+        elif o["type"] == "mesh":
+            o["translate"] = add(o.get("translate", [0.0, 0.0, 0.0]))
+        elif o["type"] == "plane":
+            o["point"] = add(o["point"])
+        else:
+            raise ValueError(o["type"])
+        objs.append(o)
+    out["objects"] = objs
+    return out

@@ -166,6 +166,29 @@ def test_f32_walk_culls_no_box_on_random_scenes(ctx, random_scene):
     assert differ == 0 and ties == 0
 
 
+def test_f32_walk_culls_no_box_far_from_origin(ctx):
+    """The same 1e4 units from the world origin. The node records hold half
+    floats in a frame centred on the scene and a ray takes its slab offsets
+    from that centre, so the boxes stay tight there; the FP32 walk's closest
+    t must still equal FP32 brute force's bit for bit on every ray (ids may
+    differ only between primitives hit at exactly the same t)."""
+    off = np.array([1.0e4, -3.0e3, 2.5e3])
+    o = orc.OracleScene(scenes.translated(scenes.random_scene_json(300, 600, 1, False), off))
+    dev = ctx.scene(o.desc)
+    try:
+        rays = ray_set(177, n=1 << 20)
+        rays[:, 0:3] += off
+        rays[:, :6] = rays[:, :6].astype(np.float32)
+        differ, ties = f32_walk_vs_brute(ctx, dev, rays)
+        h, st = ctx.trace(dev, rays, mode=0, use_bvh=1, precision=rtg.PRECISION_F32)
+        print(f"far scene: {int((h['flags'] & 1).sum())} hits, {st.node_visits / len(rays):.2f} node boxes per ray; "
+              f"walk vs brute: {differ} differ, {ties} ties", flush=True)
+        assert int((h["flags"] & 1).sum()) > 100000
+        assert differ == 0 and ties <= 1e-4 * len(rays)
+    finally:
+        dev.release()
+
+
 def test_f32_cornell_scale_rays(ctx):
     """At the 555-unit scale of the Cornell workload, slab distances carry an
     absolute rounding error of ~|o/d| 2^-24, large next to the distance to a

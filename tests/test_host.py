@@ -132,7 +132,27 @@ def test_rt_cli_cpu_paths():
 
 
 NODE_SCENES = [("cornell_c1.json", None), ("torus_c2.json", None), ("sphere_field_c3.json", None), (None, 1),
-               (None, 2), (None, 3)]
+               (None, 2), (None, 3), ("far", 1), ("huge", 3)]
+
+
+def node_scene_json(name, seed):
+    """A shipped scene, a random scene (tests/test_bvh.cpp:11-30), or a
+    random scene moved 1e4 units from the origin ("far") or scaled up past
+    what half floats hold even in the node frame ("huge": its boxes become
+    infinite there, which still bounds them)."""
+    if name == "far":
+        return scenes.translated(scenes.random_scene_json(300, 600, seed, False), (1.0e4, -3.0e3, 2.5e3))
+    if name == "huge":
+        j = scenes.random_scene_json(300, 600, seed, False)
+        f = 1.0e12
+        for o in j["objects"]:
+            for k in ("center", "v0", "v1", "v2", "point"):
+                if k in o:
+                    o[k] = [c * f for c in o[k]]
+            if "radius" in o:
+                o["radius"] *= f
+        return j
+    return scenes.load(name) if name else scenes.random_scene_json(300, 600, seed, seed == 2)
 
 
 @pytest.mark.parametrize("name,seed", NODE_SCENES)
@@ -142,7 +162,7 @@ def test_fp32_node_boxes_contain_the_double_boxes(name, seed):
     stands for (include/raytracer/geometry/bvh.hpp:20-26) and is the tightest
     half box that does, child refs point at the right records or leaf ranges,
     and the reported split axis is the node's."""
-    j = scenes.load(name) if name else scenes.random_scene_json(300, 600, seed, seed == 2)
+    j = node_scene_json(name, seed)
     hs = rtg.HostScene(json.dumps(j))
     d = hs.desc
     nodes = np.frombuffer((rtg.BvhNode * d.n_nodes).from_address(C.addressof(d.nodes.contents)),
@@ -158,11 +178,16 @@ def test_fp32_node_boxes_contain_the_double_boxes(name, seed):
     half = rec[:, :12].astype(np.float16)
     assert np.array_equal(half.astype(np.float32), rec[:, :12])  # the planes are half floats
     world = lambda h: origin.astype(np.float64) + h.astype(np.float64) * 2.0 ** e  # exact in double
-    for child, lo, hi in ((left, half[:, 0:3], half[:, 3:6]), (right, half[:, 6:9], half[:, 9:12])):
-        assert (world(lo) <= nodes["lo"][child]).all() and (world(hi) >= nodes["hi"][child]).all()
-        # outward rounding is tight: the next half inward no longer contains the box
-        assert (world(np.nextafter(lo, np.float16(np.inf))) > nodes["lo"][child]).all()
-        assert (world(np.nextafter(hi, np.float16(-np.inf))) < nodes["hi"][child]).all()
+    with np.errstate(invalid="ignore", over="ignore"):
+        for child, lo, hi in ((left, half[:, 0:3], half[:, 3:6]), (right, half[:, 6:9], half[:, 9:12])):
+            assert (world(lo) <= nodes["lo"][child]).all() and (world(hi) >= nodes["hi"][child]).all()
+            # outward rounding is tight: the next half inward no longer contains the box
+            assert (world(np.nextafter(lo, np.float16(np.inf))) > nodes["lo"][child]).all()
+            assert (world(np.nextafter(hi, np.float16(-np.inf))) < nodes["hi"][child]).all()
+    if name == "far":  # the frame follows the scene: nothing is infinite, boxes stay tight
+        assert np.isfinite(half).all() and abs(origin[0] - 1.0e4) < 100
+    if name == "huge":
+        assert e == 20 and np.isinf(half).any()
     refs = rec[:, 12:15].view(np.uint32)
     assert np.array_equal(refs[:, 2], nodes["axis"][interior])
     for col, child in ((0, left), (1, right)):
