@@ -47,8 +47,7 @@ struct PathBuffers {
   RayArrays<T> ray[2];
   T *lx, *ly, *lz;                 // radiance (by path id)
   T *ht, *hb1, *hb2;               // hit t, barycentrics (by queue slot)
-  uint32_t* hprim;                 // hit primitive (0xFFFFFFFF none; by queue slot)
-  int32_t* hplane;                 // hit plane (-1 none; by queue slot)
+  uint32_t* hprim;                 // hit: primitive, kHitPlane | plane, or kHitNone (by queue slot)
   uint64_t *rs, *ri;               // PCG32 state/inc (FP64 mode)
   uint32_t* q[2];                  // active queues (kSubQueues sub-queues of sub_cap each)
   uint32_t* counts;                // sub-queue sizes (x kCountStride): [b*kSubQueues + s] active, [(kMaxBounce+b)*kSubQueues + s] shadow
@@ -383,13 +382,14 @@ __device__ __forceinline__ uint32_t slot_pid(const PathBuffers<T>& B, int b, uin
   return b == 0 ? slot : B.q[b & 1][slot];
 }
 
-// A closest hit for the shade kernel, which reads the distance of hits only
-// and the barycentrics of primitive hits only: a miss stores its ids, a
-// plane hit its ids and distance.
+// A closest hit for the shade kernel in one word (primitive ids stay below
+// 2^25), which reads the distance of hits only and the barycentrics of
+// primitive hits only: a miss stores its word, a plane hit its word and
+// distance.
+constexpr uint32_t kHitNone = 0xFFFFFFFFu, kHitPlane = 0x80000000u;
 template <class T, class H>
 __device__ __forceinline__ void store_hit(const PathBuffers<T>& B, uint32_t k, const H& h) {
-  B.hprim[k] = h.prim;
-  B.hplane[k] = h.plane;
+  B.hprim[k] = h.prim != 0xFFFFFFFFu ? h.prim : h.plane >= 0 ? kHitPlane | (uint32_t)h.plane : kHitNone;
   if (h.prim != 0xFFFFFFFFu || h.plane >= 0) B.ht[k] = h.t;
   if (h.prim != 0xFFFFFFFFu) {
     B.hb1[k] = h.b1;
@@ -442,8 +442,9 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
     const uint32_t k = active_slot(P, pre, b, i);
     const RayArrays<T>& R = B.ray[b & 1];
     const V3<T> d{R.dx[k], R.dy[k], R.dz[k]};
-    const uint32_t prim = B.hprim[k];
-    const int plane = B.hplane[k];
+    const uint32_t word = B.hprim[k];
+    const uint32_t prim = (word & kHitPlane) ? 0xFFFFFFFFu : word;
+    const int plane = word == kHitNone ? -1 : (word & kHitPlane) ? (int)(word & ~kHitPlane) : -1;
     pid = slot_pid(B, b, k);
     const uint32_t pixel = pid_pixel(P, pid), sample = pid_sample(P, pid);
     V3<T> beta{T(1), T(1), T(1)};
