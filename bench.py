@@ -30,10 +30,10 @@ WORKLOADS = {
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # Algorithmic bytes of the FP32 closest-hit kernel: 32 B per node box tested
 # (the per-node figure the measurement is defined with), 48 B per triangle
-# test, 24 B of ray read (origin + direction) and 20 B of hit written per
-# ray. The device's node records are 32 bytes for both children (half-float
+# test, 24 B of ray read (origin + direction) and 16 B of hit written per
+# ray (a word for the primitive or plane, t, two barycentrics). The device's node records are 32 bytes for both children (half-float
 # boxes), so a box tested moves 16 B: reported beside as the layout's bytes.
-NODE_B, TRI_B, RAY_IN_B, HIT_OUT_B, RADIANCE_OUT_B = 32, 48, 24, 20, 12
+NODE_B, TRI_B, RAY_IN_B, HIT_OUT_B, RADIANCE_OUT_B = 32, 48, 24, 16, 12
 NODE_LAYOUT_B = 16
 
 
