@@ -909,15 +909,15 @@ uint64_t lay_out_paths(char* base, uint32_t cap, bool batch, unsigned long long*
   const uint64_t c = batch ? (uint64_t)cap + kBatchQueueSlack : (uint64_t)sub_cap * kSubQueues;  // >= cap
   auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
   // two buffers of 6 ray + 3 throughput arrays of T and a specular flag;
-  // 3 L + 3 hit + 10 shadow arrays of T; 2 RNG u64; 4 u32 (2 queues, the hit
-  // word, sq_pid); counts. FP32: a second shadow queue (10 T + sq_pid) and
-  // 3 shadow-radiance arrays.
+  // 3 L + 3 hit + 10 shadow arrays of T; FP64: two buffers of PCG32 state
+  // and increment (u64); 4 u32 (2 queues, the hit word, sq_pid); counts.
+  // FP32: a second shadow queue (10 T + sq_pid) and 3 shadow-radiance arrays.
   constexpr bool f32 = sizeof(T) == 4;
   const uint64_t sq = f32 ? 2 * c : c;  // shadow-queue entries
   const uint64_t n_counts =
       (batch ? (uint64_t)kMaxBatchJobs * kBatchCounts : 2ull * kMaxBounce * kSubQueues) * kCountStride;
   const uint64_t n_stats = batch ? (uint64_t)kMaxBatchJobs * kBatchStatSlots * kStatStride : 0;
-  const uint64_t bytes = al(c * sizeof(T)) * (24 + (f32 ? 3 : 0)) + al(sq * sizeof(T)) * 10 + al(c * 8) * 2 +
+  const uint64_t bytes = al(c * sizeof(T)) * (24 + (f32 ? 3 : 0)) + al(sq * sizeof(T)) * 10 + al(c * 8) * (f32 ? 0 : 4) +
                          al(c * 4) * 3 + al(sq * 4) + al(c) * 2 + al(sizeof(uint32_t) * n_counts) + al(8 * n_stats);
   if (!base) return bytes;
   char* p = base;
@@ -940,8 +940,10 @@ uint64_t lay_out_paths(char* base, uint32_t cap, bool batch, unsigned long long*
     B->slz = reinterpret_cast<T*>(take(c * sizeof(T)));
   }
   B->sq_stride = f32 ? (uint32_t)c : 0u;
-  B->rs = reinterpret_cast<uint64_t*>(take(c * 8));
-  B->ri = reinterpret_cast<uint64_t*>(take(c * 8));
+  for (RayArrays<T>& R : B->ray) {  // FP64: the PCG32 stream of the path at each slot
+    R.rs = f32 ? nullptr : reinterpret_cast<uint64_t*>(take(c * 8));
+    R.ri = f32 ? nullptr : reinterpret_cast<uint64_t*>(take(c * 8));
+  }
   B->q[0] = reinterpret_cast<uint32_t*>(take(c * 4));
   B->q[1] = reinterpret_cast<uint32_t*>(take(c * 4));
   B->hprim = reinterpret_cast<uint32_t*>(take(c * 4));

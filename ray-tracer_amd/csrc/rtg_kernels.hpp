@@ -35,6 +35,7 @@ struct RayArrays {
   T *ox, *oy, *oz, *dx, *dy, *dz;  // the ray
   T *bx, *by, *bz;                 // throughput
   uint8_t* spec;                   // specular_bounce
+  uint64_t *rs, *ri;               // PCG32 state / increment (FP64 mode; FP32: null)
 };
 template <class T>
 struct PathBuffers {
@@ -48,7 +49,6 @@ struct PathBuffers {
   T *lx, *ly, *lz;                 // radiance (by path id)
   T *ht, *hb1, *hb2;               // hit t, barycentrics (by queue slot)
   uint32_t* hprim;                 // hit: primitive, kHitPlane | plane, or kHitNone (by queue slot)
-  uint64_t *rs, *ri;               // PCG32 state/inc (FP64 mode)
   uint32_t* q[2];                  // active queues (kSubQueues sub-queues of sub_cap each)
   uint32_t* counts;                // sub-queue sizes (x kCountStride): [b*kSubQueues + s] active, [(kMaxBounce+b)*kSubQueues + s] shadow
   uint32_t* sq_pid;                // shadow queue (same sub-queue layout)
@@ -277,24 +277,38 @@ struct ModeOps<float> {
 // Draw source of one path at one bounce.
 template <class T>
 struct DrawSrc;
+// FP64: the path's PCG32 stream, its state kept with the ray by queue slot
+// (bounce b at slot k of ray[b & 1]); a path that lives on hands it to the
+// next bounce's slot. FP32: a counter hash, nothing kept.
 template <>
 struct DrawSrc<double> {
-  __device__ static PcgDraws make(const PathBuffers<double>& B, const ChunkParams&, uint32_t pid, uint32_t, uint32_t,
+  struct Next {
+    uint64_t state = 0, inc = 0;
+  };
+  __device__ static PcgDraws make(const PathBuffers<double>& B, const ChunkParams&, int b, uint32_t k, uint32_t,
                                   uint32_t) {
+    const RayArrays<double>& R = B.ray[b & 1];
     PcgDraws d;
-    d.pcg.state = B.rs[pid];
-    d.pcg.inc = B.ri[pid];
+    d.pcg.state = R.rs[k];
+    d.pcg.inc = R.ri[k];
     return d;
   }
-  __device__ static void store(const PathBuffers<double>& B, uint32_t pid, const PcgDraws& d) { B.rs[pid] = d.pcg.state; }
+  __device__ static Next next(const PcgDraws& d) { return Next{d.pcg.state, d.pcg.inc}; }
+  __device__ static void store(const PathBuffers<double>& B, int b, uint32_t k, const Next& n) {
+    const RayArrays<double>& R = B.ray[b & 1];
+    R.rs[k] = n.state;
+    R.ri[k] = n.inc;
+  }
 };
 template <>
 struct DrawSrc<float> {
-  __device__ static HashDraws make(const PathBuffers<float>&, const ChunkParams& P, uint32_t, uint32_t pixel,
-                                   uint32_t sample, uint32_t bounce) {
-    return HashDraws(path_key(P.seed, pixel, sample), bounce);
+  struct Next {};
+  __device__ static HashDraws make(const PathBuffers<float>&, const ChunkParams& P, int b, uint32_t, uint32_t pixel,
+                                   uint32_t sample) {
+    return HashDraws(path_key(P.seed, pixel, sample), (uint32_t)b);
   }
-  __device__ static void store(const PathBuffers<float>&, uint32_t, const HashDraws&) {}
+  __device__ static Next next(const HashDraws&) { return Next{}; }
+  __device__ static void store(const PathBuffers<float>&, int, uint32_t, const Next&) {}
 };
 
 static_assert(kBlock == (int)kStackStride, "traversal stacks are laid out for kBlock threads");
@@ -351,8 +365,8 @@ __device__ __forceinline__ void generate_body(const DevScene& S, const PathBuffe
     const double jx = rng.next();
     const double jy = rng.next();
     ModeOps<T>::camera(S, x, y, jx, jy, rng, o, d);
-    B.rs[pid] = rng.pcg.state;
-    B.ri[pid] = rng.pcg.inc;
+    B.ray[0].rs[pid] = rng.pcg.state;
+    B.ray[0].ri[pid] = rng.pcg.inc;
   } else {
     HashDraws rng(path_key(P.seed, pixel, sample), 0xFFFFFFFFu);
     const float jx = rng.next();
@@ -436,8 +450,9 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
   bool alive = false, shadow = false;
   uint32_t pid = 0, shadow_cls = 0;
   ShadowReq<T> req;
-  V3<T> nro, ndir, nbeta;  // the next ray and throughput of a path that lives on
+  V3<T> nro, ndir, nbeta;  // the next ray, throughput and draws of a path that lives on
   bool nspec = false;
+  typename DrawSrc<T>::Next ndraw{};
   if (i < qv.total) {
     const uint32_t k = active_slot(P, pre, b, i);
     const RayArrays<T>& R = B.ray[b & 1];
@@ -458,7 +473,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
     } else {  // the origin and the specular flag are read by hits only
       const V3<T> o{R.ox[k], R.oy[k], R.oz[k]};
       const bool spec = b == 0 || R.spec[k] != 0;
-      auto rng = DrawSrc<T>::make(B, P, pid, pixel, sample, (uint32_t)b);
+      auto rng = DrawSrc<T>::make(B, P, b, k, pixel, sample);
       HitRec<T> rec;
       ModeOps<T>::finalize(S, o, d, B.ht[k], B.hb1[k], B.hb2[k], prim, plane, rec);
       const Mat<T> mat = load_mat<T>(S, rec.material);
@@ -497,10 +512,10 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
           ndir = sr.dir;
           nbeta = beta;
           nspec = sr.specular;
+          ndraw = DrawSrc<T>::next(rng);
           alive = true;
         }
       }
-      DrawSrc<T>::store(B, pid, rng);
     }
   }
   // Converged: compacted appends to the next active queue and the shadow queue.
@@ -516,6 +531,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const PathBuffers<
     R.dx[k] = ndir.x; R.dy[k] = ndir.y; R.dz[k] = ndir.z;
     R.bx[k] = nbeta.x; R.by[k] = nbeta.y; R.bz[k] = nbeta.z;
     R.spec[k] = nspec ? 1 : 0;
+    DrawSrc<T>::store(B, b + 1, k, ndraw);
   }
   if (shadow) {
     const int64_t si = P.q_base + a.b + (int64_t)(b & 1) * B.sq_stride;
