@@ -689,7 +689,9 @@ rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* d, rtg_scene
   std::vector<float4> mat32((size_t)d->n_materials * 2);
   for (uint32_t i = 0; i < d->n_materials; ++i) {
     const rtg_material& m = d->materials[i];
-    mat32[2 * i] = f4((float)m.c0.x, (float)m.c0.y, (float)m.c0.z, bits_f(m.kind | (m.two_sided ? 256u : 0u)));
+    const bool textured = m.kind == RTG_MAT_LAMBERTIAN && (float)m.p0 > 0.0f;  // checker (material.cpp:14-24)
+    mat32[2 * i] = f4((float)m.c0.x, (float)m.c0.y, (float)m.c0.z,
+                      bits_f(m.kind | (m.two_sided ? 256u : 0u) | (textured ? 512u : 0u)));
     mat32[2 * i + 1] = f4((float)m.c1.x, (float)m.c1.y, (float)m.c1.z, (float)m.p0);
   }
   std::vector<float4> plane32((size_t)d->n_planes * 2);

@@ -74,7 +74,7 @@ struct DevScene {
   const uint32_t* prim_slot;
   const uint8_t* sampled;
   const rtg_material* materials;
-  const float4* mat32;  // 2 per material: {c0, kind | two_sided<<8} {c1, p0}
+  const float4* mat32;  // 2 per material: {c0, kind | two_sided<<8 | checker<<9} {c1, p0}
   const rtg_plane* planes;
   const float4* plane32;  // 2 per plane: {point, material} {normal, 0}
   const rtg_light_ref* lights;

@@ -148,9 +148,10 @@ __device__ inline void finalize32(const DevScene& S, V3<float> o, V3<float> d, f
   const V3<float> ng{s0.x, s0.y, s0.z};
   set_face_normal(rec, d, ng);
   if (w & kShadeSmooth) {
-    const float4 s1 = sh[1], s2 = sh[2], s3 = sh[3];
+    const float4 s1 = sh[1], s2 = sh[2];
+    const float s3x = reinterpret_cast<const float*>(sh + 3)[0];  // the record's last used word
     const float w0 = 1.0f - b1 - b2;
-    const V3<float> n0{s1.x, s1.y, s1.z}, n1{s1.w, s2.x, s2.y}, n2{s2.z, s2.w, s3.x};
+    const V3<float> n0{s1.x, s1.y, s1.z}, n1{s1.w, s2.x, s2.y}, n2{s2.z, s2.w, s3x};
     const V3<float> ns = normalize_or(n0 * w0 + n1 * b1 + n2 * b2, ng);
     rec.normal = rec.front_face ? ns : -ns;
     if (dot(rec.normal, rec.gn) <= 0.0f) rec.normal = rec.gn;
@@ -169,9 +170,14 @@ __device__ __forceinline__ Mat<double> load_mat64(const DevScene& S, uint32_t id
   const rtg_material& m = S.materials[id];
   return Mat<double>{m.kind, m.two_sided != 0, d3(m.c0), d3(m.c1), m.p0};
 }
+// The second float4 (c1, p0) is read only where it is used: every kind but
+// an untextured Lambertian (kind word bit 9 marks a checker texture), whose
+// albedo is c0 alone (material.cpp:14-24 with p0 <= 0).
 __device__ __forceinline__ Mat<float> load_mat32(const DevScene& S, uint32_t id) {
-  const float4 a = S.mat32[2 * id], b = S.mat32[2 * id + 1];
+  const float4 a = S.mat32[2 * id];
   const uint32_t k = __float_as_uint(a.w);
+  float4 b{0.0f, 0.0f, 0.0f, 0.0f};
+  if ((k & 0xFFu) != RTG_MAT_LAMBERTIAN || (k & 512u)) b = S.mat32[2 * id + 1];
   return Mat<float>{k & 0xFFu, ((k >> 8) & 1u) != 0, V3<float>{a.x, a.y, a.z}, V3<float>{b.x, b.y, b.z}, b.w};
 }
 template <class T>
