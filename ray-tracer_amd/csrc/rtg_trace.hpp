@@ -4,16 +4,18 @@
 // (src/core/scene.cpp:139-234) and Bvh::closest / any
 // (include/raytracer/geometry/bvh.hpp:75-132):
 //   * planes are tested first, outside the BVH, and bound t_max;
-//   * closest-hit visits the near child first (by the sign of the ray
-//     direction on the node's split axis) and keeps the first strictly
-//     closer hit; any-hit returns on the first hit;
+//   * closest-hit keeps the first strictly closer hit; any-hit returns on
+//     the first hit;
 //   * node_visits counts node boxes tested, prim_tests primitive tests.
-// FP64 functions are the reference algorithm on the reference layout and
-// reproduce its hits bit for bit. FP32 functions run on the two-child
-// layout: both child boxes are tested from one 64-byte record against the
-// current closest t, and only the far child's reference is pushed; a popped
-// child is entered without a second box test (its box was already tested,
-// and any primitive in it still has to beat the closest t). The FP32 walk
+// FP64 functions are the reference algorithm on the reference layout (the
+// near child first by the sign of the direction on the node's split axis)
+// and reproduce its hits bit for bit. FP32 functions run on the two-child
+// layout: both child boxes (half floats in the scene's node frame) are
+// tested from one 32-byte record against the current closest t, the child
+// with the smaller entry distance is entered first (any-hit: the larger),
+// and only the other child's reference is pushed; a popped child is entered
+// without a second box test (its box was already tested, and any primitive
+// in it still has to beat the closest t). The FP32 walk
 // parks leaves (see Walk), so a ray may test a leaf after nodes that follow
 // it in depth-first order; every such choice depends on the ray alone.
 #pragma once
