@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -649,21 +650,24 @@ rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* d, rtg_scene
       prim_slot[i] = (uint32_t)slot_prim.size();
       slot_prim.push_back(i);
     }
-  std::vector<float4> slot32((size_t)slot_prim.size() * kSlot32Stride);
+  std::vector<float> slot32((size_t)slot_prim.size() * kSlot32Floats, 0.0f);
   std::vector<float4> shade32((size_t)np * kShade32Stride);
   for (uint32_t k = 0; k < slot_prim.size(); ++k) {
     const uint32_t i = slot_prim[k];
     const rtg_primitive& p = d->prims[i];
-    float4* o = &slot32[(size_t)k * kSlot32Stride];
+    float* o = &slot32[(size_t)k * kSlot32Floats];
     if (p.kind == RTG_PRIM_SPHERE) {
       const double r = p.v1.x;
-      o[0] = f4((float)p.v0.x, (float)p.v0.y, (float)p.v0.z, bits_f(i | kSlotSphere));
-      o[1] = f4((float)r, (float)(r * r), 0.0f, 0.0f);
-      o[2] = f4(0, 0, 0, 0);
+      const float c[9] = {(float)p.v0.x, (float)p.v0.y, (float)p.v0.z, bits_f(kSlotSphere), (float)(r * r), 0, 0, 0, 0};
+      std::copy(c, c + 9, o);
     } else {
-      o[0] = f4((float)p.v0.x, (float)p.v0.y, (float)p.v0.z, bits_f(i));
-      o[1] = f4((float)(p.v1.x - p.v0.x), (float)(p.v1.y - p.v0.y), (float)(p.v1.z - p.v0.z), 0.0f);
-      o[2] = f4((float)(p.v2.x - p.v0.x), (float)(p.v2.y - p.v0.y), (float)(p.v2.z - p.v0.z), 0.0f);
+      const float c[9] = {(float)p.v0.x,          (float)p.v0.y,          (float)p.v0.z,
+                          (float)(p.v1.x - p.v0.x), (float)(p.v1.y - p.v0.y), (float)(p.v1.z - p.v0.z),
+                          (float)(p.v2.x - p.v0.x), (float)(p.v2.y - p.v0.y), (float)(p.v2.z - p.v0.z)};
+      std::copy(c, c + 9, o);
+      // the sphere marker is a NaN payload; a NaN edge component (it never
+      // hits) is written as the default NaN
+      if (std::isnan(o[3])) o[3] = std::numeric_limits<float>::quiet_NaN();
     }
   }
   for (uint32_t i = 0; i < np; ++i) {
@@ -739,6 +743,7 @@ rtg_status rtg_scene_create(rtg_context* ctx, const rtg_scene_desc* d, rtg_scene
   if ((st = up.put((host), (n), &(dst))) != RTG_OK) return cleanup(st)
   PUT(node2.data(), node2.size(), S.node2);
   PUT(slot32.data(), slot32.size(), S.slot32);
+  PUT(slot_prim.data(), slot_prim.size(), S.slot_prim);
   PUT(shade32.data(), shade32.size(), S.shade32);
   PUT(prim_slot.data(), prim_slot.size(), S.prim_slot);
   PUT(mat32.data(), mat32.size(), S.mat32);

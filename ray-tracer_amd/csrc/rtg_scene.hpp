@@ -10,8 +10,10 @@
 //            loads per level compared with the reference's one-box nodes;
 //            half floats halve the bytes each level moves through the
 //            vector memory pipeline, which bounds the traversal kernels.
-//   slot32 — primitives in leaf order (slot k = prim_indices[k]), 48 bytes:
-//            triangle {v0, id} {e1} {e2}; sphere {centre, id} {r, r^2}.
+//   slot32 — primitives in leaf order (slot k = prim_indices[k]), nine
+//            floats (36 bytes): triangle {v0} {e1} {e2}; sphere {centre}
+//            {kSlotSphere, r^2, 0} {0}. slot_prim maps a slot to its
+//            primitive id (read once per ray, for the closest hit).
 // FP64 traversal data (the exact mode): the reference's own node and
 //   primitive arrays, uploaded unchanged.
 // Shading data is indexed by primitive id.
@@ -41,7 +43,10 @@ __host__ __device__ inline uint32_t leaf_ref(uint32_t first, uint32_t count) {
 __host__ __device__ inline uint32_t leaf_first(uint32_t ref) { return ref & 0x1FFFFFFu; }
 __host__ __device__ inline uint32_t leaf_count(uint32_t ref) { return ((ref >> 25) & 63u) + 1; }
 
-constexpr uint32_t kSlotSphere = 0x80000000u;  // set in slot32[.].w bits for spheres
+// A sphere slot's fourth word: a quiet NaN whose payload no triangle edge
+// carries (the host writes a triangle's NaN edge component as the default
+// NaN, which tests the same: it never hits).
+constexpr uint32_t kSlotSphere = 0x7FC5F1E3u;
 
 // shade32 flags in the material word
 constexpr uint32_t kShadeSmooth = 1u << 30;
@@ -50,13 +55,14 @@ constexpr uint32_t kShadeSphere = 1u << 31;
 enum : uint32_t { kL32Point = 0, kL32Sphere = 1, kL32Triangle = 2 };
 constexpr int kLight32Stride = 5;  // float4 per light
 constexpr int kShade32Stride = 4;  // float4 per primitive
-constexpr int kSlot32Stride = 3;   // float4 per slot
+constexpr int kSlot32Floats = 9;   // floats per slot
 constexpr int kNode2Stride = 2;    // float4 per interior node
 
 struct DevScene {
   // FP32 traversal
   const float4* node2;
-  const float4* slot32;
+  const float* slot32;
+  const uint32_t* slot_prim;  // slot -> primitive id
   uint32_t root_ref;  // child ref of the root (interior: 0)
   uint32_t has_bvh;
   float root_lo[3], root_hi[3];  // the root box in the node frame (half values)

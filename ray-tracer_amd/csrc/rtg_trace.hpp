@@ -369,21 +369,25 @@ __device__ __forceinline__ uint32_t ld_global(const uint32_t* p) {
   return *(const __attribute__((address_space(1))) uint32_t*)p;
 }
 
-// One primitive slot's three float4, loaded together so that a triangle
-// test waits for memory once. Spheres do not read the third float4 (their
-// slot pads it), and the compiler would sink its load into the triangle
-// branch, after the wait for the first two: a second round trip per
-// triangle. The empty asm statement takes it as an operand before any
-// branch (measured: closest-hit 3 % faster on the torus workload).
+// One primitive slot's nine floats (36 bytes: three dword-aligned vec3
+// loads), loaded together so that a triangle test waits for memory once.
+// Spheres do not read the third vec3 (their slot pads it), and the compiler
+// would sink its load into the triangle branch, after the wait for the first
+// two: a second round trip per triangle. The empty asm statement takes it as
+// an operand before any branch (measured: closest-hit 3 % faster on the
+// torus workload).
+typedef float NativeF3 __attribute__((ext_vector_type(3)));
+typedef NativeF3 NativeF3u __attribute__((aligned(4)));
 struct Slot32 {
-  float4 a, b, c;
+  NativeF3 a, b, c;
 };
-__device__ __forceinline__ Slot32 load_slot32(const float4* __restrict__ slot32, uint32_t k) {
-  // A 32-bit byte offset (k < 2^25 slots of 48 bytes) from the array base:
+__device__ __forceinline__ Slot32 load_slot32(const float* __restrict__ slot32, uint32_t k) {
+  // A 32-bit byte offset (k < 2^25 slots of 36 bytes) from the array base:
   // one address add per load group.
-  const float4* p = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(slot32) +
-                                                    k * (uint32_t)(kSlot32Stride * sizeof(float4)));
-  const Slot32 s{ld_global(p), ld_global(p + 1), ld_global(p + 2)};
+  // (a vec3 type is 16 bytes in memory: the vectors sit 12 bytes apart)
+  typedef const __attribute__((address_space(1))) NativeF3u* G3;
+  const char* p = reinterpret_cast<const char*>(slot32) + k * (uint32_t)(kSlot32Floats * sizeof(float));
+  const Slot32 s{*(G3)p, *(G3)(p + 12), *(G3)(p + 24)};
   asm volatile("" ::"v"(s.c.x));
   return s;
 }
@@ -391,11 +395,9 @@ __device__ __forceinline__ Slot32 load_slot32(const float4* __restrict__ slot32,
 // Slot test (triangle: Möller–Trumbore on precomputed edges; sphere: the
 // stable quadratic). Writes t, barycentrics when closer than tmax.
 __device__ __forceinline__ bool slot_test32(const Slot32& s, const Ray32& r, float tmin, float tmax, float& t_out,
-                                            float& b1o, float& b2o, uint32_t& prim) {
-  const float4 a = s.a, b = s.b, c = s.c;
-  const uint32_t id = __float_as_uint(a.w);
-  prim = id & 0x7FFFFFFFu;
-  if (!(id & kSlotSphere)) {
+                                            float& b1o, float& b2o) {
+  const NativeF3 a = s.a, b = s.b, c = s.c;
+  if (__float_as_uint(b.x) != kSlotSphere) {
     const V3<float> d{r.dx, r.dy, r.dz};
     const V3<float> e1{b.x, b.y, b.z}, e2{c.x, c.y, c.z};
     const V3<float> pvec = cross(d, e2);
@@ -415,7 +417,7 @@ __device__ __forceinline__ bool slot_test32(const Slot32& s, const Ray32& r, flo
     }
     return ok;
   }
-  // sphere: centre a.xyz, radius b.x, r^2 b.y
+  // sphere: centre a, r^2 b.y
   const V3<float> d{r.dx, r.dy, r.dz};
   const V3<float> oc{a.x - r.ox, a.y - r.oy, a.z - r.oz};
   const float aa = len2(d);
@@ -442,9 +444,9 @@ __device__ __forceinline__ bool slot_test32(const Slot32& s, const Ray32& r, flo
   return true;
 }
 
-__device__ __forceinline__ bool slot_hit32(const float4* __restrict__ slot32, uint32_t k, const Ray32& r, float tmin,
-                                           float tmax, float& t_out, float& b1o, float& b2o, uint32_t& prim) {
-  return slot_test32(load_slot32(slot32, k), r, tmin, tmax, t_out, b1o, b2o, prim);
+__device__ __forceinline__ bool slot_hit32(const float* __restrict__ slot32, uint32_t k, const Ray32& r, float tmin,
+                                           float tmax, float& t_out, float& b1o, float& b2o) {
+  return slot_test32(load_slot32(slot32, k), r, tmin, tmax, t_out, b1o, b2o);
 }
 
 __device__ __forceinline__ int planes32(const DevScene& S, const Ray32& r, float tmin, float& tmax) {
@@ -599,8 +601,9 @@ __device__ __forceinline__ bool closest_begin32(const DevScene& S, const Ray32& 
   T.w.park(stk);
   return true;
 }
-// One round of the closest-hit walk; true when the walk has ended. A lane
-// with no walk (ref and leaf kNoRef) passes through.
+// One round of the closest-hit walk; true when the walk has ended (h final:
+// the hit's slot is then mapped to its primitive id). Called only while the
+// walk runs: from a true closest_begin32 until it returns true.
 __device__ __forceinline__ bool closest_round32(const DevScene& S, const Ray32& r, float tmin, LdsStack stk,
                                                 Counters& c, Trav32& T) {
   Walk& w = T.w;
@@ -620,18 +623,19 @@ __device__ __forceinline__ bool closest_round32(const DevScene& S, const Ray32& 
       const Slot32 cur = load_slot32(S.slot32, k);
       ++c.tests;
       float t, b1, b2;
-      uint32_t prim;
-      if (slot_test32(cur, r, tmin, h.t, t, b1, b2, prim)) {
+      if (slot_test32(cur, r, tmin, h.t, t, b1, b2)) {
         h.t = t;
         h.b1 = b1;
         h.b2 = b2;
-        h.prim = prim;
+        h.prim = k;  // the slot until the walk ends
       }
     }
     w.leaf = kNoRef;
     w.park(stk);
   }
-  return w.ref == kNoRef;
+  if (w.ref != kNoRef) return false;
+  if (h.prim != 0xFFFFFFFFu) h.prim = ld_global(S.slot_prim + h.prim);  // slot -> primitive id, once per ray
+  return true;
 }
 
 __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& r, float tmin, float tmax,
@@ -645,12 +649,11 @@ __device__ inline Hit32 closest32(const DevScene& S, bool use_bvh, const Ray32& 
     for (uint32_t i = 0; i < S.n_prims; ++i) {
       ++c.tests;
       float t, b1, b2;
-      uint32_t prim;
-      if (slot_hit32(S.slot32, ld_global(S.prim_slot + i), r, tmin, h.t, t, b1, b2, prim)) {
+      if (slot_hit32(S.slot32, ld_global(S.prim_slot + i), r, tmin, h.t, t, b1, b2)) {
         h.t = t;
         h.b1 = b1;
         h.b2 = b2;
-        h.prim = prim;
+        h.prim = i;
       }
     }
     return h;
@@ -707,8 +710,7 @@ __device__ __forceinline__ bool any_round32(const DevScene& S, const Ray32& r, f
       const Slot32 cur = load_slot32(S.slot32, k);
       ++c.tests;
       float t, b1, b2;
-      uint32_t prim;
-      if (slot_test32(cur, r, tmin, tmax, t, b1, b2, prim)) {
+      if (slot_test32(cur, r, tmin, tmax, t, b1, b2)) {
         hit = true;
         w.ref = w.leaf = kNoRef;
         return true;
@@ -731,10 +733,9 @@ __device__ inline bool any32(const DevScene& S, bool use_bvh, const Ray32& r, fl
       if (tmin < t && t < tmax) return true;
     }
     float t, b1, b2;
-    uint32_t prim;
     for (uint32_t i = 0; i < S.n_prims; ++i) {
       ++c.tests;
-      if (slot_hit32(S.slot32, ld_global(S.prim_slot + i), r, tmin, tmax, t, b1, b2, prim)) return true;
+      if (slot_hit32(S.slot32, ld_global(S.prim_slot + i), r, tmin, tmax, t, b1, b2)) return true;
     }
     return false;
   }
