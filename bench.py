@@ -31,10 +31,12 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # Algorithmic bytes of the FP32 closest-hit kernel: 32 B per node box tested
 # (the per-node figure the measurement is defined with), 48 B per triangle
 # test, 24 B of ray read (origin + direction) and 16 B of hit written per
-# ray (a word for the primitive or plane, t, two barycentrics). The device's node records are 32 bytes for both children (half-float
-# boxes), so a box tested moves 16 B: reported beside as the layout's bytes.
+# ray (a word for the primitive or plane, t, two barycentrics). The device's
+# node records are 32 bytes for both children (half-float boxes), so a box
+# tested moves 16 B, and its triangle slots are 36 bytes (v0 and two edges):
+# reported beside as the layout's bytes.
 NODE_B, TRI_B, RAY_IN_B, HIT_OUT_B, RADIANCE_OUT_B = 32, 48, 24, 16, 12
-NODE_LAYOUT_B = 16
+NODE_LAYOUT_B, TRI_LAYOUT_B = 16, 36
 
 
 def load_pkg():
@@ -146,7 +148,8 @@ def extend_roofline(prof, traffic, kind="extend"):
                        (RAY_IN_B + HIT_OUT_B) * prof["closest_rays"])
     per_launch = bytes_total / n_launch
     visits = prof["closest_visits"] + (prof["shadow_visits"] if kind == "direct" else 0)
-    layout_per_launch = (bytes_total - (NODE_B - NODE_LAYOUT_B) * visits) / n_launch
+    tests = prof["closest_tests"] + (prof["shadow_tests"] if kind == "direct" else 0)
+    layout_per_launch = (bytes_total - (NODE_B - NODE_LAYOUT_B) * visits - (TRI_B - TRI_LAYOUT_B) * tests) / n_launch
     avg_ms = ext["ms"] / n_launch
     achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     layout_achieved = layout_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
